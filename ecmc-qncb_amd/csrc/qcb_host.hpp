@@ -1,0 +1,135 @@
+// qcb_host.hpp -- host-side handle types of the C-ABI (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/qcb.h"
+#include "qcb_internal.hpp"
+
+struct qcb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+};
+
+namespace qcb {
+enum Family { kFamHalf = 0, kFamSite = 1, kFamLink = 2 };
+}
+
+struct qcb_lattice {
+    qcb_ctx* ctx = nullptr;
+    int prec = QCB_F64;
+    int L[4] = {0, 0, 0, 0};
+    uint32_t V = 0, Vh = 0, S = 0;
+    int esize = 8;
+    void* U = nullptr;
+    void* F[3] = {nullptr, nullptr, nullptr};
+    void* d_desc = nullptr;
+    uint32_t* d_blk[3] = {nullptr, nullptr, nullptr};
+    uint32_t nb[3] = {0, 0, 0};
+    double* d_partial = nullptr;   // 2 * nb[kFamSite] doubles
+    double* d_out = nullptr;       // 8 doubles of results
+    unsigned long long* d_counter = nullptr;
+    double beta = 0.0;
+    uint64_t seed = 0;
+    uint32_t gstep = 0, flow_start = 0;
+    bool flow_active = false;   // flow buffers hold a GradientFlow state (qcb_flow_copy)
+};
+
+int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
+
+#define QCB_HIP(ctx, expr)                                                                                  \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            return qcb_set_error((ctx), QCB_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                 __FILE__, __LINE__);                                                       \
+    } while (0)
+
+namespace qcb {
+
+inline uint64_t field_bytes(const qcb_lattice* l) { return (uint64_t)8 * 18 * l->S * (uint64_t)l->esize; }
+
+inline uint32_t fam_units(const qcb_lattice* l, int fam) {
+    return fam == kFamHalf ? l->Vh : fam == kFamSite ? 2 * l->Vh : 8 * l->Vh;
+}
+
+template <class T>
+inline LatDev<T> make_desc(const qcb_lattice* l) {
+    LatDev<T> d{};
+    d.U = (T*)l->U;
+    for (int k = 0; k < 3; ++k) d.F[k] = (T*)l->F[k];
+    for (int k = 0; k < 4; ++k) d.L[k] = l->L[k];
+    d.Lxh = l->L[0] / 2;
+    d.Vh = l->Vh;
+    d.S = l->S;
+    d.beta = l->beta;
+    d.key0 = (uint32_t)l->seed;
+    d.key1 = (uint32_t)(l->seed >> 32);
+    d.flow_start = l->flow_start;
+    return d;
+}
+
+inline int upload_desc(qcb_lattice* l) {
+    if (l->prec == QCB_F64) {
+        LatDev<double> d = make_desc<double>(l);
+        QCB_HIP(l->ctx, hipMemcpyAsync(l->d_desc, &d, sizeof d, hipMemcpyHostToDevice, l->ctx->stream));
+    } else {
+        LatDev<float> d = make_desc<float>(l);
+        QCB_HIP(l->ctx, hipMemcpyAsync(l->d_desc, &d, sizeof d, hipMemcpyHostToDevice, l->ctx->stream));
+    }
+    // the host copy is a stack temporary: wait for the copy before returning
+    QCB_HIP(l->ctx, hipStreamSynchronize(l->ctx->stream));
+    return QCB_OK;
+}
+
+inline Launch lat_launch(const qcb_lattice* l, int fam) {
+    Launch g;
+    g.lats = l->d_desc;
+    g.blk = l->d_blk[fam];
+    g.n_lat = 1;
+    g.nblocks = l->nb[fam];
+    g.stream = l->ctx->stream;
+    return g;
+}
+
+inline int lattice_alloc(qcb_lattice* l) {
+    qcb_ctx* c = l->ctx;
+    if (hipMalloc(&l->U, field_bytes(l)) != hipSuccess)
+        return qcb_set_error(c, QCB_ENOMEM, "cannot allocate %llu bytes of links", (unsigned long long)field_bytes(l));
+    QCB_HIP(c, hipMalloc(&l->d_desc, sizeof(LatDev<double>)));
+    for (int f = 0; f < 3; ++f) {
+        l->nb[f] = (fam_units(l, f) + kBlock - 1) / kBlock;
+        uint32_t h[2] = {0, l->nb[f]};
+        QCB_HIP(c, hipMalloc((void**)&l->d_blk[f], sizeof h));
+        QCB_HIP(c, hipMemcpy(l->d_blk[f], h, sizeof h, hipMemcpyHostToDevice));
+    }
+    QCB_HIP(c, hipMalloc((void**)&l->d_partial, 2 * sizeof(double) * l->nb[kFamSite]));
+    QCB_HIP(c, hipMalloc((void**)&l->d_out, 8 * sizeof(double)));
+    QCB_HIP(c, hipMalloc((void**)&l->d_counter, sizeof(unsigned long long)));
+    QCB_HIP(c, hipMemset(l->d_counter, 0, sizeof(unsigned long long)));
+    return upload_desc(l);
+}
+
+inline int ensure_flow_buffers(qcb_lattice* l) {
+    if (l->F[0]) return QCB_OK;
+    for (int k = 0; k < 3; ++k)
+        if (hipMalloc(&l->F[k], field_bytes(l)) != hipSuccess)
+            return qcb_set_error(l->ctx, QCB_ENOMEM, "cannot allocate flow buffer %d (%llu bytes)", k,
+                                 (unsigned long long)field_bytes(l));
+    return upload_desc(l);
+}
+
+inline void lattice_free(qcb_lattice* l) {
+    (void)hipFree(l->U);
+    for (int k = 0; k < 3; ++k) (void)hipFree(l->F[k]);
+    (void)hipFree(l->d_desc);
+    for (int f = 0; f < 3; ++f) (void)hipFree(l->d_blk[f]);
+    (void)hipFree(l->d_partial);
+    (void)hipFree(l->d_out);
+    (void)hipFree(l->d_counter);
+}
+
+}  // namespace qcb

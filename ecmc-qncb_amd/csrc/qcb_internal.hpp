@@ -1,0 +1,50 @@
+// qcb_internal.hpp -- host-side launchers shared by the C-ABI implementation
+// and the kernel translation units (not part of the public interface).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qcb_layout.hpp"
+
+namespace qcb {
+
+// Work decomposition of one launch over a batch of lattices.  Each kernel
+// family has its own per-lattice unit count: cb sites of one parity (Vh),
+// sites (2*Vh) or links (8*Vh); blk holds the n_lat+1 prefix sums of
+// ceil(units / kBlock) for that family.
+struct Launch {
+    const void* lats;       // device LatDev<T>[n_lat]
+    const uint32_t* blk;    // device prefix sums for the family being launched
+    int n_lat;
+    uint32_t nblocks;       // blk[n_lat]
+    hipStream_t stream;
+};
+
+enum HbMode { kHbPhilox = 0, kHbInjected = 1 };
+
+template <class T> void launch_cold(const Launch& g);
+template <class T> void launch_hot(const Launch& g);   // key from descriptor
+template <class T> void launch_aos_to_soa(const Launch& g, const double* aos, T* dst_field);
+template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
+template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
+template <class T> void launch_plaquette(const Launch& g, double* partial);
+template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
+template <class T> void launch_staples(const Launch& g, T* out_field);
+template <class T> void launch_hb_half(const Launch& g, int mu, int parity, int n_hits, uint32_t sweep_idx,
+                                       int mode, const double* inj, int K, unsigned long long* exhausted);
+template <class T> void launch_reunit(const Launch& g);
+template <class T> void launch_translate(const Launch& g, const int s[4], T* dst_field);
+// One RK3 stage on every lattice of the batch.  Field roles follow the
+// per-lattice step parity (gstep - flow_start) & 1; which_src = 0 reads U for
+// stage 1 of a lattice's first step when flowing the links in place.
+template <class T> void launch_flow_stage(const Launch& g, int stage, uint32_t gstep, double eps);
+template <class T> void launch_math_probe(int op, int n, const double* in, double param, int aux, double* out,
+                                          hipStream_t st);
+
+// Fixed-order per-lattice reductions of per-block partials.
+// plaq: out[l*ostride] = sum / (6 V);  qe: out[l*ostride] = Q, out[l*ostride+1] = E / V
+template <class T> void launch_finalize_plaq(const Launch& g, const double* partial, double* out, int ostride);
+template <class T> void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, double* out,
+                                           int ostride);
+
+}  // namespace qcb

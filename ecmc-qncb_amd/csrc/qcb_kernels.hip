@@ -1,0 +1,799 @@
+// qcb_kernels.hip -- gfx950 kernels of the heatbath / Wilson flow / observable
+// path, templated on the link type (float or double).  Every kernel runs over
+// a batch of lattices described by LatDev entries; a block belongs to exactly
+// one lattice, so per-lattice results do not depend on what else is batched.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "qcb_internal.hpp"
+#include "qcb_layout.hpp"
+#include "qcb_rng.hpp"
+#include "qcb_su3.hpp"
+
+namespace qcb {
+
+namespace {
+
+template <class T>
+struct BlockCtx {
+    const LatDev<T>* L;
+    uint32_t local;   // block index within its lattice
+    uint32_t tile;    // global tile index (owner of partial slot)
+};
+
+template <class T>
+__device__ __forceinline__ BlockCtx<T> block_ctx(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                 int n_lat, uint32_t nblocks) {
+    BlockCtx<T> c;
+    c.tile = xcd_tile(blockIdx.x, nblocks);
+    int l = find_lat(blk, n_lat, c.tile);
+    c.L = lats + l;
+    c.local = c.tile - blk[l];
+    return c;
+}
+
+// --- block reduction of doubles (wave64 shuffles, then LDS across 4 waves) ---
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < kBlock / 64; ++k) r += lds[k];
+    }
+    __syncthreads();
+    return r;   // valid on thread 0
+}
+
+// ---------------------------------------------------------------- fills ----
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_cold(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                 int n_lat, uint32_t nblocks) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    M3<T> m;
+    m3_identity(m);
+    st_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i, m);
+}
+
+// Haar SU(3) from Philox: 18 Box-Muller Gaussians, modified Gram-Schmidt on
+// the columns, then det^(-1/3).  All in double, stored as T.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_hot(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                int n_lat, uint32_t nblocks) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    Coords c = cb_coords(L, i, p);
+    uint32_t lex = (uint32_t)c.x + (uint32_t)L.L[0] * ((uint32_t)c.y + (uint32_t)L.L[1] * ((uint32_t)c.z + (uint32_t)L.L[2] * (uint32_t)c.t));
+    double g[18];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        U4 o = philox4x32_10(U4{lex, (uint32_t)mu, 1u, (uint32_t)k}, L.key0, L.key1);
+        double u1 = u53(o.x, o.y), u2 = u53(o.z, o.w);
+        double r = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(2.0 * M_PI * u2, &sn, &cs);
+        g[2 * k] = r * cs;
+        g[2 * k + 1] = r * sn;
+    }
+    M3<double> q;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) q.v[k] = g[k];
+    // modified Gram-Schmidt over columns 0,1,2
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+#pragma unroll
+        for (int pc = 0; pc < col; ++pc) {
+            double dr = 0, di = 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                dr += q.re(r, pc) * q.re(r, col) + q.im(r, pc) * q.im(r, col);
+                di += q.re(r, pc) * q.im(r, col) - q.im(r, pc) * q.re(r, col);
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                double a = q.re(r, pc) * dr - q.im(r, pc) * di;
+                double b = q.re(r, pc) * di + q.im(r, pc) * dr;
+                q.re(r, col) -= a;
+                q.im(r, col) -= b;
+            }
+        }
+        double n = 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) n += q.re(r, col) * q.re(r, col) + q.im(r, col) * q.im(r, col);
+        n = 1.0 / sqrt(n);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) { q.re(r, col) *= n; q.im(r, col) *= n; }
+    }
+    // det by cofactors along row 0
+    double c0r = (q.re(1, 1) * q.re(2, 2) - q.im(1, 1) * q.im(2, 2)) - (q.re(1, 2) * q.re(2, 1) - q.im(1, 2) * q.im(2, 1));
+    double c0i = (q.re(1, 1) * q.im(2, 2) + q.im(1, 1) * q.re(2, 2)) - (q.re(1, 2) * q.im(2, 1) + q.im(1, 2) * q.re(2, 1));
+    double c1r = (q.re(1, 2) * q.re(2, 0) - q.im(1, 2) * q.im(2, 0)) - (q.re(1, 0) * q.re(2, 2) - q.im(1, 0) * q.im(2, 2));
+    double c1i = (q.re(1, 2) * q.im(2, 0) + q.im(1, 2) * q.re(2, 0)) - (q.re(1, 0) * q.im(2, 2) + q.im(1, 0) * q.re(2, 2));
+    double c2r = (q.re(1, 0) * q.re(2, 1) - q.im(1, 0) * q.im(2, 1)) - (q.re(1, 1) * q.re(2, 0) - q.im(1, 1) * q.im(2, 0));
+    double c2i = (q.re(1, 0) * q.im(2, 1) + q.im(1, 0) * q.re(2, 1)) - (q.re(1, 1) * q.im(2, 0) + q.im(1, 1) * q.re(2, 0));
+    double dre = q.re(0, 0) * c0r - q.im(0, 0) * c0i + q.re(0, 1) * c1r - q.im(0, 1) * c1i + q.re(0, 2) * c2r - q.im(0, 2) * c2i;
+    double dim = q.re(0, 0) * c0i + q.im(0, 0) * c0r + q.re(0, 1) * c1i + q.im(0, 1) * c1r + q.re(0, 2) * c2i + q.im(0, 2) * c2r;
+    double ph = -atan2(dim, dre) / 3.0;
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    M3<T> out;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        out.v[2 * k] = (T)(q.v[2 * k] * cs - q.v[2 * k + 1] * sn);
+        out.v[2 * k + 1] = (T)(q.v[2 * k] * sn + q.v[2 * k + 1] * cs);
+    }
+    st_link(L.U, L.S, mu, p, i, out);
+}
+
+// ------------------------------------------------------- AoS <-> SoA -------
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_aos_to_soa(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                       int n_lat, uint32_t nblocks, const double* __restrict__ aos,
+                                                       T* __restrict__ dst) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    Coords c = cb_coords(L, i, p);
+    size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
+    const double* src = aos + (lex * 4 + (size_t)mu) * 18;
+    M3<T> m;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) m.v[k] = (T)src[k];
+    st_link(dst, L.S, mu, p, i, m);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_soa_to_aos(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                       int n_lat, uint32_t nblocks, const T* __restrict__ src,
+                                                       double* __restrict__ aos) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    Coords c = cb_coords(L, i, p);
+    size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
+    double* d = aos + (lex * 4 + (size_t)mu) * 18;
+    M3<T> m = ld_link(src, L.S, mu, p, i);
+#pragma unroll
+    for (int k = 0; k < 18; ++k) d[k] = (double)m.v[k];
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                       int n_lat, uint32_t nblocks, const T* __restrict__ src,
+                                                       T* __restrict__ dst) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    st_link(dst, L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(src, L.S, (int)(hs >> 1), (int)(hs & 1), i));
+}
+
+// U'(x) = U(x - s): each thread writes one destination link.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_translate(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                      int n_lat, uint32_t nblocks, int s0, int s1, int s2, int s3,
+                                                      T* __restrict__ dst) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    Coords c = cb_coords(L, i, p);
+    int x = c.x - s0; x += (x < 0) ? L.L[0] : 0;
+    int y = c.y - s1; y += (y < 0) ? L.L[1] : 0;
+    int z = c.z - s2; z += (z < 0) ? L.L[2] : 0;
+    int t = c.t - s3; t += (t < 0) ? L.L[3] : 0;
+    int ps = (x + y + z + t) & 1;
+    st_link(dst, L.S, mu, p, i, ld_link(L.U, L.S, mu, ps, cb_index(L, x, y, z, t)));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_reunit(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                   int n_lat, uint32_t nblocks) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    M3<T> m = ld_link(L.U, L.S, mu, p, i);
+    M3<double> d;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) d.v[k] = (double)m.v[k];
+    reunitarize_d(d);
+#pragma unroll
+    for (int k = 0; k < 18; ++k) m.v[k] = (T)d.v[k];
+    st_link(L.U, L.S, mu, p, i, m);
+}
+
+// ------------------------------------------------------------ plaquette ----
+
+// Per-site sum over mu<nu of Re Tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag)/3
+// (reference src/observables/observables_mpi.cpp:8-30), FP64 accumulation.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                      int n_lat, uint32_t nblocks, double* __restrict__ partial) {
+    __shared__ double lds[kBlock / 64];
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    double sum = 0.0;
+    if (j < 2u * L.Vh) {
+        int p = j >= L.Vh;
+        uint32_t i = j - (p ? L.Vh : 0u);
+        Coords c = cb_coords(L, i, p);
+        const T* f = L.U;
+#pragma unroll
+        for (int mu = 0; mu < 3; ++mu) {
+            M3<T> umu = ld_link(f, L.S, mu, p, i);
+            const uint32_t xmu = nb_cb(L, c, mu, +1);
+#pragma unroll
+            for (int nu = mu + 1; nu < 4; ++nu) {
+                const uint32_t xnu = nb_cb(L, c, nu, +1);
+                M3<T> a = mul_nn(umu, ld_link(f, L.S, nu, p ^ 1, xmu));
+                M3<T> b = mul_nn(ld_link(f, L.S, nu, p, i), ld_link(f, L.S, mu, p ^ 1, xnu));
+                sum += (double)retrace_nd(a, b) / 3.0;
+            }
+        }
+    }
+    double r = block_sum(sum, lds);
+    if (threadIdx.x == 0) partial[bc.tile] = r;
+}
+
+// --------------------------------------------------------------- clover ----
+
+// F_{mu nu}(x) = (C - C^dag)/8 of the four-leaf clover, not made traceless
+// (reference src/observables/observables_mpi.cpp:61-85).
+template <class T, int MU, int NU>
+__device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i) {
+    const uint32_t S = L.S;
+    const int q = p ^ 1;
+    const uint32_t xpmu = nb_cb(L, c, MU, +1), xpnu = nb_cb(L, c, NU, +1);
+    const uint32_t xmmu = nb_cb(L, c, MU, -1), xmnu = nb_cb(L, c, NU, -1);
+    const uint32_t xmmupnu = nb_cb(L, c, MU, -1, NU, +1), xmmumnu = nb_cb(L, c, MU, -1, NU, -1);
+    const uint32_t xpmumnu = nb_cb(L, c, MU, +1, NU, -1);
+    M3<T> C;
+    {   // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+        M3<T> umu = ld_link(f, S, MU, p, i);
+        M3<T> a = mul_nn(umu, ld_link(f, S, NU, q, xpmu));
+        M3<T> b = mul_nn(ld_link(f, S, NU, p, i), ld_link(f, S, MU, q, xpnu));
+        C = mul_nd(a, b);
+    }
+    {   // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
+        M3<T> a = mul_nd(ld_link(f, S, NU, p, i), ld_link(f, S, MU, p, xmmupnu));
+        M3<T> b = mul_dn(ld_link(f, S, MU, q, xmmu), ld_link(f, S, NU, q, xmmu));
+        acc_nd(C, a, b);
+    }
+    {   // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+        M3<T> a = mul_nn(ld_link(f, S, NU, p, xmmumnu), ld_link(f, S, MU, q, xmmu));
+        M3<T> b = mul_nn(ld_link(f, S, MU, p, xmmumnu), ld_link(f, S, NU, q, xmnu));
+        acc_dn(C, a, b);
+    }
+    {   // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
+        M3<T> a = mul_nn(ld_link(f, S, MU, q, xmnu), ld_link(f, S, NU, p, xpmumnu));
+        M3<T> b = mul_dn(ld_link(f, S, NU, q, xmnu), a);
+        acc_nd(C, b, ld_link(f, S, MU, p, i));
+        // C now holds all four leaves (acc_nd adds b * U_mu(x)^dag)
+    }
+    M3<T> F;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            F.re(a, b) = T(0.125) * (C.re(a, b) - C.re(b, a));
+            F.im(a, b) = T(0.125) * (C.im(a, b) + C.im(b, a));
+        }
+    return F;
+}
+
+template <class T>
+__device__ __forceinline__ double sqnorm(const M3<T>& a) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) s += (double)a.v[k] * (double)a.v[k];
+    return s;
+}
+
+// q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
+// (reference src/observables/observables_mpi.cpp:88-107).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                   int n_lat, uint32_t nblocks, int which, uint32_t gstep,
+                                                   double* __restrict__ pq, double* __restrict__ pe) {
+    __shared__ double lds[kBlock / 64];
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    double q = 0.0, e = 0.0;
+    if (j < 2u * L.Vh) {
+        int p = j >= L.Vh;
+        uint32_t i = j - (p ? L.Vh : 0u);
+        Coords c = cb_coords(L, i, p);
+        const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
+        double tq = 0.0;
+        {
+            M3<T> a = clover_F<T, 0, 1>(L, f, c, p, i);
+            M3<T> b = clover_F<T, 2, 3>(L, f, c, p, i);
+            tq += (double)retrace_nn(a, b);
+            e += sqnorm(a) + sqnorm(b);
+        }
+        {
+            M3<T> a = clover_F<T, 0, 2>(L, f, c, p, i);
+            M3<T> b = clover_F<T, 1, 3>(L, f, c, p, i);
+            tq -= (double)retrace_nn(a, b);
+            e += sqnorm(a) + sqnorm(b);
+        }
+        {
+            M3<T> a = clover_F<T, 0, 3>(L, f, c, p, i);
+            M3<T> b = clover_F<T, 1, 2>(L, f, c, p, i);
+            tq += (double)retrace_nn(a, b);
+            e += sqnorm(a) + sqnorm(b);
+        }
+        q = tq * (1.0 / (4.0 * M_PI * M_PI));
+        e *= 0.5;
+    }
+    double rq = block_sum(q, lds);
+    double re = block_sum(e, lds);
+    if (threadIdx.x == 0) {
+        pq[bc.tile] = rq;
+        pe[bc.tile] = re;
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_finalize_plaq(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                          const double* __restrict__ partial, double* __restrict__ out,
+                                                          int ostride) {
+    __shared__ double lds[kBlock / 64];
+    const int l = blockIdx.x;
+    double s = 0.0;
+    for (uint32_t b = blk[l] + threadIdx.x; b < blk[l + 1]; b += kBlock) s += partial[b];
+    double r = block_sum(s, lds);
+    if (threadIdx.x == 0) out[(size_t)l * ostride] = r / (6.0 * 2.0 * (double)lats[l].Vh);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_finalize_qe(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                        const double* __restrict__ pq, const double* __restrict__ pe,
+                                                        double* __restrict__ out, int ostride) {
+    __shared__ double lds[kBlock / 64];
+    const int l = blockIdx.x;
+    double s = 0.0, t = 0.0;
+    for (uint32_t b = blk[l] + threadIdx.x; b < blk[l + 1]; b += kBlock) {
+        s += pq[b];
+        t += pe[b];
+    }
+    double rq = block_sum(s, lds);
+    double re = block_sum(t, lds);
+    if (threadIdx.x == 0) {
+        out[(size_t)l * ostride] = rq;
+        out[(size_t)l * ostride + 1] = re / (2.0 * (double)lats[l].Vh);
+    }
+}
+
+// -------------------------------------------------------------- staples ----
+
+template <class T, int MU>
+__device__ __forceinline__ void staple_one(const LatDev<T>& L, T* out, Coords c, int p, uint32_t i) {
+    st_link(out, L.S, MU, p, i, staple<T, MU>(L, L.U, c, p, i));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                    int n_lat, uint32_t nblocks, T* __restrict__ out) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    Coords c = cb_coords(L, i, p);
+    switch (mu) {
+        case 0: staple_one<T, 0>(L, out, c, p, i); break;
+        case 1: staple_one<T, 1>(L, out, c, p, i); break;
+        case 2: staple_one<T, 2>(L, out, c, p, i); break;
+        default: staple_one<T, 3>(L, out, c, p, i); break;
+    }
+}
+
+// ------------------------------------------------------------- heatbath ----
+
+__device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
+__device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
+
+// One Kennedy-Pendleton attempt (reference src/heatbath/heatbath.cpp:14-26).
+template <class T>
+__device__ __forceinline__ bool kp_try(T alpha, T delta, T r1, T r2, T r3, T& a0) {
+    T cs = cos(T(2.0 * M_PI) * r2);
+    T lam2 = T(-0.5) / alpha * (log(r1) + cs * cs * log(r3));
+    if (delta * delta <= T(1) - lam2) {
+        a0 = T(1) - T(2) * lam2;
+        return true;
+    }
+    return false;
+}
+
+// Sphere draw of the remaining three components (reference src/heatbath/heatbath.cpp:29-35).
+template <class T>
+__device__ __forceinline__ void kp_sphere(T a0, T uphi, T ucos, T x[4]) {
+    T rs = sqrt(T(1) - a0 * a0);
+    T phi = T(2.0 * M_PI) * uphi;
+    T ct = T(2) * ucos - T(1);
+    T st = sqrt(T(1) - ct * ct);
+    T sp, cp;
+    sin_cos(phi, &sp, &cp);
+    x[0] = a0;
+    x[1] = rs * st * cp;
+    x[2] = rs * st * sp;
+    x[3] = rs * ct;
+}
+
+constexpr uint32_t kPhiloxMaxAttempts = 1024;
+
+// Cabibbo-Marinari hit on U with staple A, subgroups (0,1),(1,2),(0,2)
+// (reference src/heatbath/heatbath_mpi.cpp:11-34).  Only the four entries of
+// W = U A in the subgroup block are formed, and only rows i, j of U change.
+// Returns the number of exhausted KP budgets.
+template <class T, int MODE>
+__device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, int h, uint32_t cb, uint32_t sweep,
+                                      uint32_t key0, uint32_t key1, const double* __restrict__ inj, int n_hits, int K,
+                                      uint32_t Vh) {
+    int exhausted = 0;
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+        const int si = g == 1 ? 1 : 0;
+        const int sj = g == 0 ? 1 : 2;
+        // W_rc = sum_k U_rk A_kc for r, c in {si, sj}
+        T w[8];
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const int r = rr ? sj : si, col = cc ? sj : si;
+                T re = T(0), im = T(0);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    re = fma(U.re(r, k), A.re(k, col), re);
+                    re = fma(-U.im(r, k), A.im(k, col), re);
+                    im = fma(U.re(r, k), A.im(k, col), im);
+                    im = fma(U.im(r, k), A.re(k, col), im);
+                }
+                w[(rr * 2 + cc) * 2] = re;
+                w[(rr * 2 + cc) * 2 + 1] = im;
+            }
+        // su2_to_quaternion (reference src/su3/utils.cpp:47-50)
+        T q0 = (w[0] + w[6]) / T(2), q1 = (w[3] + w[5]) / T(2);
+        T q2 = (w[2] - w[4]) / T(2), q3 = (w[1] - w[7]) / T(2);
+        T k = sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        T v0 = q0 / k, v1 = q1 / k, v2 = q2 / k, v3 = q3 / k;
+        T alpha = beta * k * T(2) / T(3);
+        T x[4];
+        bool ok = false;
+        if (MODE == kHbInjected) {
+            const int nslot = 4 * K + 2;
+            const double* u = inj + ((size_t)((hs * n_hits + h) * 3 + g) * (size_t)nslot) * Vh + cb;
+            T a0 = T(0);
+            for (int a = 0; a < K; ++a) {
+                if (kp_try<T>(alpha, (T)u[(size_t)(4 * a) * Vh], (T)u[(size_t)(4 * a + 1) * Vh],
+                              (T)u[(size_t)(4 * a + 2) * Vh], (T)u[(size_t)(4 * a + 3) * Vh], a0)) {
+                    kp_sphere<T>(a0, (T)u[(size_t)(4 * K) * Vh], (T)u[(size_t)(4 * K + 1) * Vh], x);
+                    ok = true;
+                    break;
+                }
+            }
+        } else {
+            const uint32_t cz = hb_ctr_z(hs, h, g);
+            T a0 = T(0);
+            for (uint32_t a = 0; a < kPhiloxMaxAttempts; ++a) {
+                U4 o0 = philox4x32_10(U4{cb, sweep, cz, 2 * a}, key0, key1);
+                U4 o1 = philox4x32_10(U4{cb, sweep, cz, 2 * a + 1}, key0, key1);
+                if (kp_try<T>(alpha, (T)u53(o0.x, o0.y), (T)u53(o0.z, o0.w), (T)u53(o1.x, o1.y), (T)u53(o1.z, o1.w),
+                              a0)) {
+                    U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
+                    kp_sphere<T>(a0, (T)u53(os.x, os.y), (T)u53(os.z, os.w), x);
+                    ok = true;
+                    break;
+                }
+            }
+        }
+        if (!ok) {
+            ++exhausted;
+            continue;
+        }
+        // r = mult(x, adj(v)) (reference src/heatbath/heatbath.cpp:39-44, src/su3/utils.cpp:53-63)
+        const T p0 = v0, p1 = -v1, p2 = -v2, p3 = -v3;
+        T r0 = x[0] * p0 - x[1] * p1 - x[2] * p2 - x[3] * p3;
+        T r1 = x[0] * p1 + x[1] * p0 + x[2] * p3 - x[3] * p2;
+        T r2 = x[0] * p2 - x[1] * p3 + x[2] * p0 + x[3] * p1;
+        T r3 = x[0] * p3 + x[1] * p2 - x[2] * p1 + x[3] * p0;
+        // U <- R U with R embedded in (si, sj) (reference src/su3/utils.cpp:33-44)
+        // R_ii = r0 + i r3, R_ij = r2 + i r1, R_ji = -r2 + i r1, R_jj = r0 - i r3
+#pragma unroll
+        for (int col = 0; col < 3; ++col) {
+            T uir = U.re(si, col), uii = U.im(si, col), ujr = U.re(sj, col), uji = U.im(sj, col);
+            T ni_r = r0 * uir - r3 * uii + (r2 * ujr - r1 * uji);
+            T ni_i = r0 * uii + r3 * uir + (r2 * uji + r1 * ujr);
+            T nj_r = -r2 * uir - r1 * uii + (r0 * ujr + r3 * uji);
+            T nj_i = -r2 * uii + r1 * uir + (r0 * uji - r3 * ujr);
+            U.re(si, col) = ni_r;
+            U.im(si, col) = ni_i;
+            U.re(sj, col) = nj_r;
+            U.im(sj, col) = nj_i;
+        }
+    }
+    return exhausted;
+}
+
+// One (mu, parity) half-sweep: each thread owns one link U_mu(x), computes its
+// staple once and applies n_hits CM hits in registers.
+template <class T, int MU, int MODE>
+__global__ __launch_bounds__(kBlock) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                    int n_lat, uint32_t nblocks, int p, int n_hits, uint32_t sweep,
+                                                    const double* __restrict__ inj, int K,
+                                                    unsigned long long* __restrict__ exhausted) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t i = bc.local * kBlock + threadIdx.x;
+    if (i >= L.Vh) return;
+    Coords c = cb_coords(L, i, p);
+    M3<T> A = staple<T, MU>(L, L.U, c, p, i);
+    M3<T> U = ld_link(L.U, L.S, MU, p, i);
+    const T beta = (T)L.beta;
+    int ex = 0;
+    for (int h = 0; h < n_hits; ++h)
+        ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sweep, L.key0, L.key1, inj, n_hits, K, L.Vh);
+    st_link(L.U, L.S, MU, p, i, U);
+    if (ex) atomicAdd(exhausted, (unsigned long long)ex);
+}
+
+// --------------------------------------------------------------- flow ------
+
+// One fused Luescher RK3 stage for one link (reference src/flow/gradient_flow.cpp:15-128):
+//  Z = P_TA(-W Sigma[W]);
+//  stage 1: Zs = Z,                      W' = exp(eps/4 Z) W
+//  stage 2: Zs = 8/9 Z - 17/36 Zs,       W' = exp(eps Zs) W
+//  stage 3:                              W' = exp(eps (3/4 Z - Zs)) W
+template <class T, int MU, int STAGE>
+__device__ __forceinline__ void flow_link(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
+                                          T* __restrict__ zs, Coords c, int p, uint32_t i, T eps) {
+    M3<T> Sg = staple<T, MU>(L, win, c, p, i);
+    M3<T> W = ld_link(win, L.S, MU, p, i);
+    M3<T> Z = mul_nn(W, Sg);
+#pragma unroll
+    for (int k = 0; k < 18; ++k) Z.v[k] = -Z.v[k];
+    proj_lie(Z);
+    M3<T> X;
+    if (STAGE == 1) {
+        st_link(zs, L.S, MU, p, i, Z);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) X.v[k] = T(0.25) * eps * Z.v[k];
+    } else if (STAGE == 2) {
+        M3<T> Z0 = ld_link((const T*)zs, L.S, MU, p, i);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) Z0.v[k] = T(8.0 / 9.0) * Z.v[k] - T(17.0 / 36.0) * Z0.v[k];
+        st_link(zs, L.S, MU, p, i, Z0);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) X.v[k] = eps * Z0.v[k];
+    } else {
+        M3<T> Zc = ld_link((const T*)zs, L.S, MU, p, i);
+#pragma unroll
+        for (int k = 0; k < 18; ++k) X.v[k] = T(0.75) * eps * Z.v[k] - eps * Zc.v[k];
+    }
+    M3<T> E = exp_su3<T, ExpTerms<T>::N>(X);
+    st_link(wout, L.S, MU, p, i, mul_nn(E, W));
+}
+
+template <class T, int STAGE>
+__global__ __launch_bounds__(kBlock) void k_flow_stage(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                       int n_lat, uint32_t nblocks, uint32_t gstep, T eps) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    uint32_t j = bc.local * kBlock + threadIdx.x;
+    if (j >= 8u * L.Vh) return;
+    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    const int cur = (int)((gstep - L.flow_start) & 1u);
+    const T* win = L.F[(STAGE == 2) ? 1 - cur : cur];
+    T* wout = L.F[(STAGE == 2) ? cur : 1 - cur];
+    T* zs = L.F[2];
+    Coords c = cb_coords(L, i, p);
+    switch (mu) {
+        case 0: flow_link<T, 0, STAGE>(L, win, wout, zs, c, p, i, eps); break;
+        case 1: flow_link<T, 1, STAGE>(L, win, wout, zs, c, p, i, eps); break;
+        case 2: flow_link<T, 2, STAGE>(L, win, wout, zs, c, p, i, eps); break;
+        default: flow_link<T, 3, STAGE>(L, win, wout, zs, c, p, i, eps); break;
+    }
+}
+
+// ---------------------------------------------------------- math probes ----
+
+template <class T>
+__global__ void k_math_probe(int op, int n, const double* __restrict__ in, double param, int aux,
+                             double* __restrict__ out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (op == 0 || op == 1 || op == 2) {
+        M3<T> a;
+        for (int k = 0; k < 18; ++k) a.v[k] = (T)in[(size_t)t * 18 + k];
+        if (op == 0) {
+            for (int k = 0; k < 18; ++k) a.v[k] = (T)param * a.v[k];
+            a = exp_su3<T, ExpTerms<T>::N>(a);
+        } else if (op == 1) {
+            proj_lie(a);
+        } else {
+            M3<double> d;
+            for (int k = 0; k < 18; ++k) d.v[k] = (double)a.v[k];
+            reunitarize_d(d);
+            for (int k = 0; k < 18; ++k) a.v[k] = (T)d.v[k];
+        }
+        for (int k = 0; k < 18; ++k) out[(size_t)t * 18 + k] = (double)a.v[k];
+    } else if (op == 3) {
+        const int K = aux, w = 1 + 4 * K + 2;
+        const double* u = in + (size_t)t * w;
+        T k = (T)u[0];
+        T alpha = (T)param * k * T(2) / T(3);
+        T a0 = T(0), x[4] = {T(0), T(0), T(0), T(0)};
+        int acc = -1;
+        for (int a = 0; a < K; ++a)
+            if (kp_try<T>(alpha, (T)u[1 + 4 * a], (T)u[2 + 4 * a], (T)u[3 + 4 * a], (T)u[4 + 4 * a], a0)) {
+                kp_sphere<T>(a0, (T)u[1 + 4 * K], (T)u[2 + 4 * K], x);
+                acc = a;
+                break;
+            }
+        out[(size_t)t * 5] = acc;
+        for (int c = 0; c < 4; ++c) out[(size_t)t * 5 + 1 + c] = (double)x[c];
+    } else if (op == 4) {
+        const int K = aux, nslot = 4 * K + 2, w = 36 + 3 * nslot;
+        const double* src = in + (size_t)t * w;
+        M3<T> U, A;
+        for (int k = 0; k < 18; ++k) { U.v[k] = (T)src[k]; A.v[k] = (T)src[18 + k]; }
+        // uniforms laid out [g][slot] with stride 1: reuse the injected path with Vh = 1
+        cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1);
+        for (int k = 0; k < 18; ++k) out[(size_t)t * 18 + k] = (double)U.v[k];
+    }
+}
+
+}  // namespace
+
+// ============================================================ launchers ====
+
+#define QCB_ARGS(g) static_cast<const LatDev<T>*>((g).lats), (g).blk, (g).n_lat, (g).nblocks
+
+template <class T>
+void launch_cold(const Launch& g) {
+    hipLaunchKernelGGL(k_cold<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+}
+template <class T>
+void launch_hot(const Launch& g) {
+    hipLaunchKernelGGL(k_hot<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+}
+template <class T>
+void launch_aos_to_soa(const Launch& g, const double* aos, T* dst) {
+    hipLaunchKernelGGL(k_aos_to_soa<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), aos, dst);
+}
+template <class T>
+void launch_soa_to_aos(const Launch& g, const T* src, double* aos) {
+    hipLaunchKernelGGL(k_soa_to_aos<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, aos);
+}
+template <class T>
+void launch_copy_field(const Launch& g, const T* src, T* dst) {
+    hipLaunchKernelGGL(k_copy_field<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, dst);
+}
+template <class T>
+void launch_plaquette(const Launch& g, double* partial) {
+    hipLaunchKernelGGL(k_plaquette<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), partial);
+}
+template <class T>
+void launch_clover(const Launch& g, int which, uint32_t gstep, double* pq, double* pe) {
+    hipLaunchKernelGGL(k_clover<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), which, gstep, pq, pe);
+}
+template <class T>
+void launch_staples(const Launch& g, T* out) {
+    hipLaunchKernelGGL(k_staples<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), out);
+}
+template <class T>
+void launch_reunit(const Launch& g) {
+    hipLaunchKernelGGL(k_reunit<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+}
+template <class T>
+void launch_translate(const Launch& g, const int s[4], T* dst) {
+    hipLaunchKernelGGL(k_translate<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), s[0], s[1], s[2], s[3],
+                       dst);
+}
+
+template <class T, int MU>
+static void hb_mu(const Launch& g, int p, int n_hits, uint32_t sweep, int mode, const double* inj, int K,
+                  unsigned long long* ex) {
+    if (mode == kHbInjected)
+        hipLaunchKernelGGL((k_hb_half<T, MU, kHbInjected>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
+                           n_hits, sweep, inj, K, ex);
+    else
+        hipLaunchKernelGGL((k_hb_half<T, MU, kHbPhilox>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
+                           n_hits, sweep, inj, K, ex);
+}
+
+template <class T>
+void launch_hb_half(const Launch& g, int mu, int p, int n_hits, uint32_t sweep, int mode, const double* inj, int K,
+                    unsigned long long* ex) {
+    switch (mu) {
+        case 0: hb_mu<T, 0>(g, p, n_hits, sweep, mode, inj, K, ex); break;
+        case 1: hb_mu<T, 1>(g, p, n_hits, sweep, mode, inj, K, ex); break;
+        case 2: hb_mu<T, 2>(g, p, n_hits, sweep, mode, inj, K, ex); break;
+        default: hb_mu<T, 3>(g, p, n_hits, sweep, mode, inj, K, ex); break;
+    }
+}
+
+template <class T>
+void launch_flow_stage(const Launch& g, int stage, uint32_t gstep, double eps) {
+    if (stage == 1)
+        hipLaunchKernelGGL((k_flow_stage<T, 1>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gstep, (T)eps);
+    else if (stage == 2)
+        hipLaunchKernelGGL((k_flow_stage<T, 2>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gstep, (T)eps);
+    else
+        hipLaunchKernelGGL((k_flow_stage<T, 3>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gstep, (T)eps);
+}
+
+template <class T>
+void launch_math_probe(int op, int n, const double* in, double param, int aux, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_math_probe<T>, dim3((n + 63) / 64), dim3(64), 0, st, op, n, in, param, aux, out);
+}
+
+template <class T>
+void launch_finalize_plaq(const Launch& g, const double* partial, double* out, int ostride) {
+    hipLaunchKernelGGL(k_finalize_plaq<T>, dim3(g.n_lat), dim3(kBlock), 0, g.stream,
+                       static_cast<const LatDev<T>*>(g.lats), g.blk, partial, out, ostride);
+}
+template <class T>
+void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, double* out, int ostride) {
+    hipLaunchKernelGGL(k_finalize_qe<T>, dim3(g.n_lat), dim3(kBlock), 0, g.stream,
+                       static_cast<const LatDev<T>*>(g.lats), g.blk, pq, pe, out, ostride);
+}
+
+#define QCB_INSTANTIATE(T)                                                                                     \
+    template void launch_cold<T>(const Launch&);                                                              \
+    template void launch_hot<T>(const Launch&);                                                               \
+    template void launch_aos_to_soa<T>(const Launch&, const double*, T*);                                     \
+    template void launch_soa_to_aos<T>(const Launch&, const T*, double*);                                     \
+    template void launch_copy_field<T>(const Launch&, const T*, T*);                                          \
+    template void launch_plaquette<T>(const Launch&, double*);                                                \
+    template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
+    template void launch_staples<T>(const Launch&, T*);                                                       \
+    template void launch_reunit<T>(const Launch&);                                                            \
+    template void launch_translate<T>(const Launch&, const int*, T*);                                         \
+    template void launch_hb_half<T>(const Launch&, int, int, int, uint32_t, int, const double*, int,           \
+                                    unsigned long long*);                                                     \
+    template void launch_flow_stage<T>(const Launch&, int, uint32_t, double);                                 \
+    template void launch_math_probe<T>(int, int, const double*, double, int, double*, hipStream_t);           \
+    template void launch_finalize_plaq<T>(const Launch&, const double*, double*, int);                        \
+    template void launch_finalize_qe<T>(const Launch&, const double*, const double*, double*, int);
+
+QCB_INSTANTIATE(float)
+QCB_INSTANTIATE(double)
+
+}  // namespace qcb

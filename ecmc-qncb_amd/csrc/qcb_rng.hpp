@@ -1,0 +1,47 @@
+// qcb_rng.hpp -- counter-based Philox4x32-10 (Salmon, Moraes, Dror, Shaw,
+// SC'11) for the device heatbath and hot start.
+//
+// Counter scheme (the oracle restates the same one):
+//   key  = { seed & 0xffffffff, seed >> 32 }
+//   heatbath:  ctr = { cb_site, sweep_idx, (hs<<24)|(hit<<8)|(subgroup<<4), call }
+//              hs = 2*mu + parity; call 2a and 2a+1 feed Kennedy-Pendleton attempt a
+//              with (delta, r1) and (r2, r3); call 0x80000000 feeds the sphere draw.
+//   hot start: ctr = { lex_site, mu, 1, call 0..8 } (nine Box-Muller pairs)
+//   uniform  = u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) + 2^-53, strictly in (0,1).
+// Every number depends only on (seed, counter), never on launch shape or on
+// other lattices in a batch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qcb {
+
+struct U4 { uint32_t x, y, z, w; };
+
+__device__ __forceinline__ U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        U4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+    uint64_t x = ((uint64_t)hi << 32) | lo;
+    return (double)(x >> 12) * 0x1.0p-52 + 0x1.0p-53;
+}
+
+__host__ __device__ __forceinline__ uint32_t hb_ctr_z(int hs, int hit, int g) {
+    return ((uint32_t)hs << 24) | ((uint32_t)hit << 8) | ((uint32_t)g << 4);
+}
+
+}  // namespace qcb

@@ -1,0 +1,240 @@
+"""Python bindings of libqcb.so (the MI355X heatbath / Wilson-flow library).
+
+Thin ctypes layer over include/qcb.h used by the tests, the benchmark and the
+flow service.  Links cross the boundary as float64 arrays shaped
+(V, 4, 3, 3, 2) in the reference's interior AoS order (x fastest).  The
+library must be built (`make -C ecmc-qncb_amd`); there is no CPU fallback:
+loading fails loudly if libqcb.so is missing, and creating a context fails if
+no GPU is visible.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libqcb.so")
+
+F64, F32 = 0, 1
+OK, EINVAL, EDEVICE, ENOMEM, EBUDGET, ESTATE = 0, -1, -2, -3, -4, -5
+PROBE_EXP, PROBE_PROJ, PROBE_REUNIT, PROBE_KP, PROBE_CM_HIT = 0, 1, 2, 3, 4
+
+_D = C.POINTER(C.c_double)
+_VP = C.c_void_p
+
+
+class QcbError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"qcb error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# (name, restype, argtypes) of every entry point declared in include/qcb.h
+SIGNATURES = [
+    ("qcb_ctx_create", C.c_int, [C.c_int, C.POINTER(_VP)]),
+    ("qcb_ctx_destroy", C.c_int, [_VP]),
+    ("qcb_last_error", C.c_char_p, [_VP]),
+    ("qcb_ctx_synchronize", C.c_int, [_VP]),
+    ("qcb_timing_begin", C.c_int, [_VP]),
+    ("qcb_timing_end", C.c_int, [_VP, _D]),
+    ("qcb_lattice_create", C.c_int, [_VP, C.POINTER(C.c_int), C.c_int, C.POINTER(_VP)]),
+    ("qcb_lattice_destroy", C.c_int, [_VP]),
+    ("qcb_lattice_bytes", C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    ("qcb_lattice_set_links", C.c_int, [_VP, _D]),
+    ("qcb_lattice_get_links", C.c_int, [_VP, _D]),
+    ("qcb_cold", C.c_int, [_VP]),
+    ("qcb_hot", C.c_int, [_VP, C.c_uint64]),
+    ("qcb_plaquette", C.c_int, [_VP, _D]),
+    ("qcb_clover_qe", C.c_int, [_VP, _D, _D]),
+    ("qcb_staples", C.c_int, [_VP, _D]),
+    ("qcb_hb_sweeps", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_uint32]),
+    ("qcb_hb_sweep_injected", C.c_int, [_VP, C.c_double, C.c_int, _D, C.c_int, C.POINTER(C.c_int64)]),
+    ("qcb_reunitarize", C.c_int, [_VP]),
+    ("qcb_translate", C.c_int, [_VP, C.POINTER(C.c_int)]),
+    ("qcb_flow_steps", C.c_int, [_VP, C.c_double, C.c_int]),
+    ("qcb_flow_copy", C.c_int, [_VP]),
+    ("qcb_flow_rk3", C.c_int, [_VP, C.c_double, C.c_int]),
+    ("qcb_flow_qe", C.c_int, [_VP, _D, _D]),
+    ("qcb_flow_get_links", C.c_int, [_VP, _D]),
+    ("qcb_topo_charge_flowed", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
+    ("qcb_math_probe", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_int, _D]),
+]
+
+
+def lib():
+    """Load libqcb.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `make -C ecmc-qncb_amd`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_D)
+
+
+class Context:
+    def __init__(self, device: int = 0):
+        self.h = _VP()
+        rc = lib().qcb_ctx_create(device, C.byref(self.h))
+        if rc != OK:
+            raise QcbError(rc, "no usable GPU (qcb_ctx_create)")
+
+    def check(self, rc):
+        if rc != OK:
+            raise QcbError(rc, lib().qcb_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            lib().qcb_ctx_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        self.check(lib().qcb_ctx_synchronize(self.h))
+
+    def timing_begin(self):
+        self.check(lib().qcb_timing_begin(self.h))
+
+    def timing_end(self) -> float:
+        ms = C.c_double()
+        self.check(lib().qcb_timing_end(self.h, C.byref(ms)))
+        return ms.value
+
+    def probe(self, prec: int, op: int, inp: np.ndarray, param: float = 0.0, aux: int = 0) -> np.ndarray:
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        n = inp.shape[0]
+        wout = 5 if op == PROBE_KP else 18
+        out = np.zeros((n, wout))
+        self.check(lib().qcb_math_probe(self.h, prec, op, n, _dp(inp.reshape(n, -1)), param, aux, _dp(out)))
+        return out
+
+
+class Lattice:
+    """A device-resident gauge field (GaugeField of the reference)."""
+
+    def __init__(self, ctx: Context, dims, prec: int = F64):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        self.prec = prec
+        self.V = int(np.prod(self.dims))
+        self.h = _VP()
+        ctx.check(lib().qcb_lattice_create(ctx.h, (C.c_int * 4)(*self.dims), prec, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().qcb_lattice_destroy(self.h)
+            self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return self.ctx.check(rc)
+
+    def set_links(self, links: np.ndarray):
+        a = np.ascontiguousarray(links)
+        if a.dtype == np.complex128:
+            a = a.view(np.float64)
+        a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        if a.size != self.V * 72:
+            raise ValueError("links must hold V*4*3*3 complex numbers")
+        self._c(lib().qcb_lattice_set_links(self.h, _dp(a)))
+
+    def get_links(self) -> np.ndarray:
+        out = np.zeros(self.V * 72)
+        self._c(lib().qcb_lattice_get_links(self.h, _dp(out)))
+        return out.view(np.complex128).reshape(self.V, 4, 3, 3)
+
+    def nbytes(self) -> int:
+        b = C.c_uint64()
+        self._c(lib().qcb_lattice_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def cold(self):
+        self._c(lib().qcb_cold(self.h))
+
+    def hot(self, seed: int):
+        self._c(lib().qcb_hot(self.h, seed))
+
+    def plaquette(self) -> float:
+        p = C.c_double()
+        self._c(lib().qcb_plaquette(self.h, C.byref(p)))
+        return p.value
+
+    def clover_qe(self):
+        q, e = C.c_double(), C.c_double()
+        self._c(lib().qcb_clover_qe(self.h, C.byref(q), C.byref(e)))
+        return q.value, e.value
+
+    def staples(self) -> np.ndarray:
+        out = np.zeros(self.V * 72)
+        self._c(lib().qcb_staples(self.h, _dp(out)))
+        return out.view(np.complex128).reshape(self.V, 4, 3, 3)
+
+    def hb_sweeps(self, beta: float, n_hits: int, n_sweeps: int, seed: int, sweep_idx: int = 0):
+        self._c(lib().qcb_hb_sweeps(self.h, beta, n_hits, n_sweeps, seed, sweep_idx))
+
+    def hb_sweep_injected(self, beta: float, n_hits: int, u: np.ndarray, K: int) -> int:
+        n = C.c_int64()
+        u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        if u.size != 8 * n_hits * 3 * (4 * K + 2) * (self.V // 2):
+            raise ValueError("uniform buffer has the wrong size")
+        self._c(lib().qcb_hb_sweep_injected(self.h, beta, n_hits, _dp(u), K, C.byref(n)))
+        return n.value
+
+    def reunitarize(self):
+        self._c(lib().qcb_reunitarize(self.h))
+
+    def translate(self, s):
+        self._c(lib().qcb_translate(self.h, (C.c_int * 4)(*[int(v) for v in s])))
+
+    def flow_steps(self, eps: float, n: int):
+        self._c(lib().qcb_flow_steps(self.h, eps, n))
+
+    # GradientFlow mirror: copy -> rk3 steps -> measure, links untouched
+    def flow_copy(self):
+        self._c(lib().qcb_flow_copy(self.h))
+
+    def flow_rk3(self, eps: float, n: int = 1):
+        self._c(lib().qcb_flow_rk3(self.h, eps, n))
+
+    def flow_qe(self):
+        q, e = C.c_double(), C.c_double()
+        self._c(lib().qcb_flow_qe(self.h, C.byref(q), C.byref(e)))
+        return q.value, e.value
+
+    def flow_links(self) -> np.ndarray:
+        out = np.zeros(self.V * 72)
+        self._c(lib().qcb_flow_get_links(self.h, _dp(out)))
+        return out.view(np.complex128).reshape(self.V, 4, 3, 3)
+
+    def topo_charge_flowed(self, eps: float, n_steps_gf: int, n_rk: int, final: bool = False):
+        tqe = np.zeros(3 * max(n_steps_gf, 1))
+        fin = np.zeros(3)
+        self._c(lib().qcb_topo_charge_flowed(self.h, eps, n_steps_gf, n_rk, _dp(tqe),
+                                             _dp(fin) if final else None))
+        t = tqe[: 3 * n_steps_gf].reshape(n_steps_gf, 3)
+        return (t, fin) if final else t

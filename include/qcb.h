@@ -1,0 +1,144 @@
+/*
+ * qcb.h -- C-ABI of the MI355X heatbath / Wilson-flow / observable library
+ * (libqcb.so).  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * Conventions
+ *  - Every call returns QCB_OK (0) or a negative QCB_E* code; the message of
+ *    the last failure on a context is qcb_last_error(ctx).
+ *  - Host link buffers use the reference's interior AoS order with host
+ *    endianness: aos[((site*4 + mu)*9 + 3*i + j)*2 + {re,im}], site = x +
+ *    Lx*(y + Ly*(z + Lz*t)) (reference src/gauge/GaugeField.h:27,49-56 and the
+ *    ILDG site order of src/io/ildg.cpp:87-106).  Host buffers are borrowed
+ *    only for the duration of a call.  Device memory belongs to the handle.
+ *  - Calls on one context are ordered on its HIP stream and are not
+ *    thread-safe; use one context per host thread.  Results returned through
+ *    host pointers are complete when the call returns.
+ *  - prec: QCB_F64 (links and arithmetic in double) or QCB_F32 (links,
+ *    staples and flow stages in float; global sums, flow time and
+ *    reunitarisation in double).
+ *  - Lattice extents are periodic, each >= 4, Lx even (even/odd split);
+ *    L^3 x T lattices are allowed (the reference is hypercubic only).
+ */
+#ifndef QCB_H
+#define QCB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QCB_OK 0
+#define QCB_EINVAL (-1)     /* bad argument */
+#define QCB_EDEVICE (-2)    /* HIP runtime / no GPU */
+#define QCB_ENOMEM (-3)     /* device allocation failed */
+#define QCB_EBUDGET (-4)    /* Kennedy-Pendleton attempt budget exhausted (links left unchanged) */
+#define QCB_ESTATE (-5)     /* call not valid in the handle's state */
+
+#define QCB_F64 0
+#define QCB_F32 1
+
+typedef struct qcb_ctx qcb_ctx;
+typedef struct qcb_lattice qcb_lattice;
+typedef struct qcb_batch qcb_batch;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces the process setup of src/bin/gauge_hb_ncb.cpp:203-226 (MPI_Init and
+ * MpiTopology, src/mpi/MpiTopology.cpp:7-24): one GPU, no ranks. */
+int qcb_ctx_create(int device, qcb_ctx** out);
+int qcb_ctx_destroy(qcb_ctx* ctx);
+const char* qcb_last_error(qcb_ctx* ctx);
+int qcb_ctx_synchronize(qcb_ctx* ctx);
+/* Device-side time of the last kernel-timing window (see qcb_timing_*), ms. */
+int qcb_timing_begin(qcb_ctx* ctx);
+int qcb_timing_end(qcb_ctx* ctx, double* ms);
+
+/* ---- lattice ------------------------------------------------------------ */
+/* GaugeField(const GeometryCB&) (src/gauge/GaugeField.h:30-43): links start
+ * at the identity. dims = {Lx, Ly, Lz, Lt}. */
+int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb_lattice** out);
+int qcb_lattice_destroy(qcb_lattice* lat);
+/* Device bytes held by the handle (links plus flow buffers once allocated). */
+int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes);
+/* Upload / download of the whole field (src/io/ildg.cpp:87-106,213-232 minus
+ * the byte swap, which stays with the caller's file codec). */
+int qcb_lattice_set_links(qcb_lattice* lat, const double* aos);
+int qcb_lattice_get_links(qcb_lattice* lat, double* aos);
+
+/* GaugeField::cold_start (src/gauge/GaugeField.cpp:23-33). */
+int qcb_cold(qcb_lattice* lat);
+/* Device hot start: Haar SU(3) per link from Philox (Box-Muller Gaussians,
+ * Gram-Schmidt, det^(-1/3)); replaces GaugeField::hot_start
+ * (src/gauge/GaugeField.cpp:10-20) where the caller does not need the host
+ * mt19937_64 stream to advance. */
+int qcb_hot(qcb_lattice* lat, uint64_t seed);
+
+/* ---- observables -------------------------------------------------------- */
+/* mpi::observables::mean_plaquette_global (src/observables/observables_mpi.cpp:34-43). */
+int qcb_plaquette(qcb_lattice* lat, double* mean_plaquette);
+/* mpi::observables::topo_q_e_clover_global (src/observables/observables_mpi.cpp:130-150):
+ * Q = total clover charge, E_mean = sum_x e_x / V. */
+int qcb_clover_qe(qcb_lattice* lat, double* Q, double* E_mean);
+/* GaugeField::compute_staple for every link (src/gauge/GaugeField.cpp:74-97),
+ * written to out_aos in the host link order (a checking aid). */
+int qcb_staples(qcb_lattice* lat, double* out_aos);
+
+/* ---- heatbath ----------------------------------------------------------- */
+/* mpi::heatbathcb::samples (src/heatbath/heatbath_mpi.cpp:59-69) on the GPU:
+ * n_sweeps sweeps of 8 (mu, parity) half-sweeps, each link receiving n_hits
+ * Cabibbo-Marinari hits (src/heatbath/heatbath_mpi.cpp:11-34) with
+ * Kennedy-Pendleton SU(2) draws (src/heatbath/heatbath.cpp:8-44).  Uniforms
+ * come from Philox keyed by seed with counter sweep_idx, sweep_idx+1, ...
+ * (a Philox stream replaces the std::mt19937_64& argument). */
+int qcb_hb_sweeps(qcb_lattice* lat, double beta, int n_hits, int n_sweeps, uint64_t seed,
+                  uint32_t sweep_idx);
+/* One sweep fed with host-supplied uniforms: u[((((2*mu+p)*n_hits + h)*3 + g)*(4K+2)
+ * + slot)*(V/2) + cb], slots 4a..4a+3 = (delta, r1, r2, r3) of attempt a < K,
+ * slots 4K, 4K+1 = (phi, cos theta) uniforms.  A (link, hit, subgroup) whose K
+ * attempts all reject is left unchanged and counted in *n_exhausted. */
+int qcb_hb_sweep_injected(qcb_lattice* lat, double beta, int n_hits, const double* u, int K,
+                          int64_t* n_exhausted);
+/* GaugeField::project_field_su3 (src/gauge/GaugeField.cpp:61-71), in double. */
+int qcb_reunitarize(qcb_lattice* lat);
+/* mpi::shift::shift_field (src/mpi/Shift.cpp:109-160), one rank: U'(x) = U(x - s).
+ * The caller draws s (random_shift, src/mpi/Shift.cpp:163-186). */
+int qcb_translate(qcb_lattice* lat, const int s[4]);
+
+/* ---- Wilson flow -------------------------------------------------------- */
+/* n RK3 steps of GradientFlow::rk3_step (src/flow/gradient_flow.cpp:117-128)
+ * applied to the lattice's own links. */
+int qcb_flow_steps(qcb_lattice* lat, double eps, int n);
+/* The GradientFlow object of the reference (src/flow/gradient_flow.h:11-27)
+ * held inside the lattice handle: flow_copy = GradientFlow::copy (field_c =
+ * field, src/flow/gradient_flow.cpp:131-133); flow_rk3 = n x rk3_step
+ * (src/flow/gradient_flow.cpp:117-128) on field_c; flow_qe =
+ * topo_q_e_clover_global(field_c); flow_get_links downloads field_c.  The
+ * lattice's own links are untouched by these four calls. */
+int qcb_flow_copy(qcb_lattice* lat);
+int qcb_flow_rk3(qcb_lattice* lat, double eps, int n);
+int qcb_flow_qe(qcb_lattice* lat, double* Q, double* E_mean);
+int qcb_flow_get_links(qcb_lattice* lat, double* aos);
+/* mpi::observables::topo_charge_flowed (src/observables/observables_mpi.cpp:153-178):
+ * on a copy of the links, for s < n_steps_gf record (t, Q, E_mean) into
+ * tQE[3*s..3*s+2] and then advance n_rk_steps RK3 steps; the links are not
+ * modified.  If final_QE is non-NULL it receives (t, Q, E_mean) of the state
+ * after the last step (the reference never measures it). */
+int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk_steps,
+                           double* tQE, double* final_tQE);
+
+/* ---- device math probes (checking aids for the register-level routines) ---- */
+#define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out */
+#define QCB_PROBE_PROJ 1       /* out = P_TA(in) (utils.cpp:130-134) */
+#define QCB_PROBE_REUNIT 2     /* out = Gram-Schmidt(in) (GaugeField.cpp:36-58) */
+#define QCB_PROBE_KP 3         /* in = {k, u[4K+2]} per item, param = beta, K = (int)aux;
+                                  out = {accepted attempt or -1, x0..x3} (heatbath.cpp:8-36) */
+#define QCB_PROBE_CM_HIT 4     /* in = {U 18, A 18, u[3*(4K+2)]}, param = beta, aux = K:
+                                  out = U after one CM hit (heatbath_mpi.cpp:11-34) */
+int qcb_math_probe(qcb_ctx* ctx, int prec, int op, int n, const double* in, double param, int aux,
+                   double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -551,7 +551,8 @@ static int kp_phx_cb(void* vctx, int g, double beta, double k, double out[4]) {
     phx_ctx* c = (phx_ctx*)vctx;
     double alpha = beta * k * 2.0 / 3.0, a0;
     uint32_t c2 = (c->hs << 24) | ((uint32_t)c->h << 8) | ((uint32_t)g << 4);
-    for (uint32_t a = 0;; ++a) {
+    uint32_t a;
+    for (a = 0; a < 1024; ++a) {   /* same attempt cap as the device kernel */
         uint32_t ctr0[4] = {c->cb, c->sweep, c2, 2 * a}, ctr1[4] = {c->cb, c->sweep, c2, 2 * a + 1};
         uint32_t o0[4], o1[4];
         qo_philox4x32_10(ctr0, c->key, o0);
@@ -560,6 +561,7 @@ static int kp_phx_cb(void* vctx, int g, double beta, double k, double out[4]) {
                       qo_u53(o1[2], o1[3]), &a0))
             break;
     }
+    if (a == 1024) return -1;
     uint32_t cs[4] = {c->cb, c->sweep, c2, 0x80000000u}, os[4];
     qo_philox4x32_10(cs, c->key, os);
     qo_kp_sphere(a0, qo_u53(os[0], os[1]), qo_u53(os[2], os[3]), out);

@@ -1,0 +1,37 @@
+"""The C-ABI library loads and exports every symbol include/qcb.h declares
+(no compute calls: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import qcb
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "qcb.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(qcb_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = qcb.lib()
+    names = declared_symbols()
+    assert len(names) > 20
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_binding_table_matches_header():
+    assert sorted(n for n, _, _ in qcb.SIGNATURES) == declared_symbols()
+
+
+def test_context_creation_reports_missing_gpu():
+    """Without a GPU, qcb_ctx_create fails with QCB_EDEVICE (no CPU fallback)."""
+    try:
+        c = qcb.Context(0)
+    except qcb.QcbError as e:
+        assert e.code == qcb.EDEVICE
+    else:
+        c.close()

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path through the C-ABI against the FP64 oracle.
+
+Tolerances: FP64 kernels agree with the oracle to 1e-12 relative (the
+north-star bar for staples, plaquette, flow, Q, E and injected-uniform
+heatbath sweeps); Q is compared in absolute terms scaled by E*V because it is
+near zero.  FP32 kernels are held to float-rounding bounds stated per test.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as qo
+import qcb
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qcb.Context(0)
+    yield c
+    c.close()
+
+
+def hot_pair(ctx, dims, seed, prec=qcb.F64):
+    g = qcb.Lattice(ctx, dims, prec)
+    g.hot(seed)
+    o = qo.Lattice(dims)
+    o.hot_philox(seed)
+    return g, o
+
+
+def maxrel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_roundtrip_and_cold(ctx):
+    dims = (4, 6, 4, 8)
+    g = qcb.Lattice(ctx, dims)
+    assert g.plaquette() == 1.0
+    assert g.clover_qe() == (0.0, 0.0)
+    rng = np.random.default_rng(0)
+    U = rng.normal(size=(g.V, 4, 3, 3)) + 1j * rng.normal(size=(g.V, 4, 3, 3))
+    g.set_links(U)
+    assert np.array_equal(g.get_links(), U)
+    f = qcb.Lattice(ctx, dims, qcb.F32)
+    f.set_links(U)
+    assert np.array_equal(f.get_links(), U.astype(np.complex64).astype(np.complex128))
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 6)])
+def test_hot_start_matches_oracle(ctx, dims):
+    g, o = hot_pair(ctx, dims, 7)
+    assert maxrel(g.get_links(), o.links) < 1e-13
+    U = g.get_links().reshape(-1, 3, 3)
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-13
+    assert np.abs(np.linalg.det(U) - 1).max() < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 6, 8, 6)])
+def test_plaquette_clover_staples(ctx, dims):
+    g, o = hot_pair(ctx, dims, 11)
+    # a few oracle sweeps make a non-trivial, partly ordered field
+    for s in range(2):
+        o.sweep_philox(6.0, 1, 3, s)
+    g.set_links(o.links)
+    assert abs(g.plaquette() - o.plaquette()) < REL * abs(o.plaquette())
+    Qg, Eg = g.clover_qe()
+    Qo, Eo = o.clover_qe()
+    assert abs(Eg - Eo) < REL * Eo
+    assert abs(Qg - Qo) < REL * Eo * g.V
+    st = g.staples()
+    want = np.stack([[o.staple(s, mu) for mu in range(4)] for s in range(g.V)])
+    assert maxrel(st, want) < REL
+
+
+def test_abelian_flux_closed_form(ctx):
+    from test_oracle_lattice import abelian_field
+    dims = (6, 4, 4, 8)
+    U, f, gg = abelian_field(dims, (1, -1, 0), (2, -1, -1))
+    g = qcb.Lattice(ctx, dims)
+    g.set_links(U)
+    V = g.V
+    P = (np.cos(f).sum() / 3 + np.cos(gg).sum() / 3 + 4) / 6
+    assert abs(g.plaquette() - P) < 1e-14
+    Q, E = g.clover_qe()
+    assert abs(Q + V * np.sum(np.sin(f) * np.sin(gg)) / (4 * np.pi ** 2)) < 1e-12
+    assert abs(E - 0.5 * np.sum(np.sin(f) ** 2 + np.sin(gg) ** 2)) < 1e-14
+
+
+def _su3_algebra(rng, n, scale):
+    A = rng.normal(size=(n, 3, 3)) + 1j * rng.normal(size=(n, 3, 3))
+    Z = 0.5 * (A - A.conj().transpose(0, 2, 1))
+    Z -= np.trace(Z, axis1=1, axis2=2)[:, None, None] / 3 * np.eye(3)
+    return scale * Z
+
+
+def test_probe_exp_proj_reunit(ctx):
+    rng = np.random.default_rng(1)
+    Z = _su3_algebra(rng, 64, 0.4)
+    flat = Z.view(np.float64).reshape(64, 18)
+    out = ctx.probe(qcb.F64, qcb.PROBE_EXP, flat, 0.7).view(np.complex128).reshape(64, 3, 3)
+    want = np.stack([qo.exp_luscher(z, 0.7) for z in Z])
+    assert np.abs(out - want).max() < 1e-14
+    out32 = ctx.probe(qcb.F32, qcb.PROBE_EXP, flat, 0.7).view(np.complex128).reshape(64, 3, 3)
+    assert np.abs(out32 - want).max() < 2e-6
+    A = (rng.normal(size=(64, 3, 3)) + 1j * rng.normal(size=(64, 3, 3)))
+    out = ctx.probe(qcb.F64, qcb.PROBE_PROJ, A.view(np.float64).reshape(64, 18)).view(np.complex128).reshape(64, 3, 3)
+    assert np.abs(out - np.stack([qo.proj_lie(a) for a in A])).max() < 1e-15
+    out = ctx.probe(qcb.F64, qcb.PROBE_REUNIT, A.view(np.float64).reshape(64, 18)).view(np.complex128).reshape(64, 3, 3)
+    for a, b in zip(A, out):
+        x = qo.m3(a)
+        qo.lib().qo_reunitarize_link(qo.dp(x))
+        assert np.abs(qo.c3(x) - b).max() < 1e-14
+
+
+def test_probe_kp_and_cm_hit(ctx):
+    rng = np.random.default_rng(2)
+    K = 6
+    n = 512
+    inp = np.concatenate([rng.uniform(0.05, 3.0, size=(n, 1)), rng.uniform(size=(n, 4 * K + 2))], 1)
+    out = ctx.probe(qcb.F64, qcb.PROBE_KP, inp, 6.0, K)
+    for row, o in zip(inp, out):
+        x = np.zeros(4)
+        a = qo.lib().qo_kp_injected(6.0, row[0], qo.dp(np.ascontiguousarray(row[1:])), K, qo.dp(x))
+        assert a == int(o[0])
+        if a >= 0:
+            assert np.abs(o[1:] - x).max() < 1e-14
+
+
+def _uniforms(rng, Vh, n_hits, K):
+    return rng.uniform(size=8 * n_hits * 3 * (4 * K + 2) * Vh)
+
+
+@pytest.mark.parametrize("dims,n_hits", [((4, 4, 4, 4), 1), ((8, 8, 8, 8), 2), ((4, 6, 4, 8), 2)])
+def test_heatbath_injected_uniforms(ctx, dims, n_hits):
+    """One GPU sweep fed the same uniforms as the oracle reproduces its links to 1e-12."""
+    g, o = hot_pair(ctx, dims, 5)
+    K = 4
+    rng = np.random.default_rng(3)
+    for sweep in range(2):
+        u = _uniforms(rng, g.V // 2, n_hits, K)
+        ex_g = g.hb_sweep_injected(6.0, n_hits, u, K)
+        ex_o = o.sweep_injected(6.0, n_hits, u, K)
+        assert ex_g == ex_o
+        assert maxrel(g.get_links(), o.links) < REL
+    assert abs(g.plaquette() - o.plaquette()) < REL
+
+
+def test_heatbath_philox_matches_oracle(ctx):
+    dims = (4, 4, 6, 4)
+    g, o = hot_pair(ctx, dims, 9)
+    g.hb_sweeps(5.9, 2, 3, seed=123, sweep_idx=10)
+    for s in range(3):
+        o.sweep_philox(5.9, 2, 123, 10 + s)
+    assert maxrel(g.get_links(), o.links) < REL
+
+
+def test_heatbath_deterministic_and_su3(ctx):
+    a = qcb.Lattice(ctx, (8, 8, 8, 8), qcb.F32)
+    b = qcb.Lattice(ctx, (8, 8, 8, 8), qcb.F32)
+    for l in (a, b):
+        l.hot(4)
+        l.hb_sweeps(6.0, 2, 5, seed=99)
+    assert np.array_equal(a.get_links(), b.get_links())
+    U = a.get_links().reshape(-1, 3, 3)
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-5
+
+
+def test_reunitarize_and_translate(ctx):
+    dims = (4, 6, 4, 4)
+    g, o = hot_pair(ctx, dims, 13)
+    rng = np.random.default_rng(4)
+    M = o.links + 1e-4 * (rng.normal(size=o.links.shape) + 1j * rng.normal(size=o.links.shape))
+    o.set_links(M)
+    g.set_links(M)
+    g.reunitarize()
+    o.reunitarize()
+    assert maxrel(g.get_links(), o.links) < 1e-14
+    g.translate((1, 5, 2, 3))
+    o.translate((1, 5, 2, 3))
+    assert np.array_equal(g.get_links(), o.links)
+
+
+def test_flow_matches_oracle(ctx):
+    dims = (6, 4, 4, 6)
+    g, o = hot_pair(ctx, dims, 17)
+    for s in range(2):
+        o.sweep_philox(6.0, 1, 8, s)
+    g.set_links(o.links)
+    tg, fin = g.topo_charge_flowed(0.02, 4, 2, final=True)
+    to = o.topo_flowed(0.02, 4, 2)
+    assert np.array_equal(tg[:, 0], to[:, 0])
+    E = to[:, 2]
+    assert np.all(np.abs(tg[:, 2] - E) < REL * E)
+    assert np.all(np.abs(tg[:, 1] - to[:, 1]) < REL * E * g.V)
+    # the links themselves after the same number of RK3 steps
+    g.flow_steps(0.02, 8)
+    for _ in range(8):
+        o.flow_step(0.02)
+    assert maxrel(g.get_links(), o.links) < REL
+    assert abs(fin[0] - 0.16) < 1e-15 and abs(fin[2] - o.clover_qe()[1]) < REL * fin[2]
