@@ -48,6 +48,8 @@ extern "C" int qcb_ctx_create(int device, qcb_ctx** out) {
 
 extern "C" int qcb_ctx_destroy(qcb_ctx* ctx) {
     if (!ctx) return QCB_EINVAL;
+    if (ctx->n_live > 0)
+        return qcb_set_error(ctx, QCB_ESTATE, "%d lattice/batch handles still alive on this context", ctx->n_live);
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipEventDestroy(ctx->ev0);
@@ -57,7 +59,14 @@ extern "C" int qcb_ctx_destroy(qcb_ctx* ctx) {
     return QCB_OK;
 }
 
-extern "C" const char* qcb_last_error(qcb_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+thread_local std::string g_free_error;   // failures of context-free calls
+
+int qcb_set_free_error(int code, const std::string& msg) {
+    g_free_error = msg;
+    return code;
+}
+
+extern "C" const char* qcb_last_error(qcb_ctx* ctx) { return ctx ? ctx->err.c_str() : g_free_error.c_str(); }
 
 extern "C" int qcb_ctx_synchronize(qcb_ctx* ctx) {
     if (!ctx) return QCB_EINVAL;
@@ -113,6 +122,7 @@ extern "C" int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb
         delete l;
         return rc;
     }
+    ctx->n_live++;
     *out = l;
     return QCB_OK;
 }
@@ -121,8 +131,15 @@ extern "C" int qcb_lattice_destroy(qcb_lattice* lat) {
     if (!lat) return QCB_EINVAL;
     (void)hipSetDevice(lat->ctx->device);
     (void)hipStreamSynchronize(lat->ctx->stream);
+    lat->ctx->n_live--;
     lattice_free(lat);
     delete lat;
+    return QCB_OK;
+}
+
+int qcb_lattice_dims_internal(qcb_lattice* lat, int dims[4]) {
+    if (!lat) return QCB_EINVAL;
+    for (int d = 0; d < 4; ++d) dims[d] = lat->L[d];
     return QCB_OK;
 }
 

@@ -12,6 +12,7 @@ struct qcb_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
+    int n_live = 0;   // lattices and batches created on this context and not yet destroyed
 };
 
 namespace qcb {

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -61,6 +62,10 @@ SIGNATURES = [
     ("qcb_flow_qe", C.c_int, [_VP, _D, _D]),
     ("qcb_flow_get_links", C.c_int, [_VP, _D]),
     ("qcb_topo_charge_flowed", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
+    ("qcb_ildg_write", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
+    ("qcb_ildg_read", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
+    ("qcb_ildg_save", C.c_int, [_VP, C.c_char_p]),
+    ("qcb_ildg_load", C.c_int, [_VP, C.c_char_p]),
     ("qcb_math_probe", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_int, _D]),
 ]
 
@@ -89,6 +94,7 @@ def _dp(a: np.ndarray):
 class Context:
     def __init__(self, device: int = 0):
         self.h = _VP()
+        self._children = weakref.WeakSet()   # handles that must die before the context
         rc = lib().qcb_ctx_create(device, C.byref(self.h))
         if rc != OK:
             raise QcbError(rc, "no usable GPU (qcb_ctx_create)")
@@ -100,7 +106,9 @@ class Context:
 
     def close(self):
         if self.h:
-            lib().qcb_ctx_destroy(self.h)
+            for child in list(self._children):
+                child.close()
+            self.check(lib().qcb_ctx_destroy(self.h))
             self.h = _VP()
 
     def __del__(self):
@@ -139,11 +147,12 @@ class Lattice:
         self.V = int(np.prod(self.dims))
         self.h = _VP()
         ctx.check(lib().qcb_lattice_create(ctx.h, (C.c_int * 4)(*self.dims), prec, C.byref(self.h)))
+        ctx._children.add(self)
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().qcb_lattice_destroy(self.h)
-            self.h = _VP()
+        self.h = _VP()
 
     def __del__(self):
         try:
@@ -230,6 +239,12 @@ class Lattice:
         out = np.zeros(self.V * 72)
         self._c(lib().qcb_flow_get_links(self.h, _dp(out)))
         return out.view(np.complex128).reshape(self.V, 4, 3, 3)
+
+    def ildg_save(self, path: str):
+        self._c(lib().qcb_ildg_save(self.h, str(path).encode()))
+
+    def ildg_load(self, path: str):
+        self._c(lib().qcb_ildg_load(self.h, str(path).encode()))
 
     def topo_charge_flowed(self, eps: float, n_steps_gf: int, n_rk: int, final: bool = False):
         tqe = np.zeros(3 * max(n_steps_gf, 1))

@@ -47,9 +47,12 @@ typedef struct qcb_batch qcb_batch;
 /* Replaces the process setup of src/bin/gauge_hb_ncb.cpp:203-226 (MPI_Init and
  * MpiTopology, src/mpi/MpiTopology.cpp:7-24): one GPU, no ranks. */
 int qcb_ctx_create(int device, qcb_ctx** out);
+/* Fails with QCB_ESTATE (and frees nothing) while lattices or batches created
+ * on the context are still alive. */
 int qcb_ctx_destroy(qcb_ctx* ctx);
 const char* qcb_last_error(qcb_ctx* ctx);
 int qcb_ctx_synchronize(qcb_ctx* ctx);
+/* qcb_last_error(NULL) reports the last failure of a context-free call. */
 /* Device-side time of the last kernel-timing window (see qcb_timing_*), ms. */
 int qcb_timing_begin(qcb_ctx* ctx);
 int qcb_timing_end(qcb_ctx* ctx, double* ms);
@@ -126,6 +129,17 @@ int qcb_flow_get_links(qcb_lattice* lat, double* aos);
  * after the last step (the reference never measures it). */
 int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk_steps,
                            double* tQE, double* final_tQE);
+
+/* ---- ILDG / LIME configuration files ----------------------------------- */
+/* The file layout of save_ildg_clime / read_ildg_clime (src/io/ildg.cpp:13-237):
+ * an "ildg-format" XML record, then "ildg-binary-data" with big-endian FP64
+ * links in the host AoS order.  The codec is this library's own (LIME spec);
+ * no c-lime.  qcb_ildg_write/read need no GPU; their failure message is
+ * qcb_last_error(NULL). */
+int qcb_ildg_write(const char* path, const int dims[4], const double* aos);
+int qcb_ildg_read(const char* path, const int dims[4], double* aos);
+int qcb_ildg_save(qcb_lattice* lat, const char* path);
+int qcb_ildg_load(qcb_lattice* lat, const char* path);
 
 /* ---- device math probes (checking aids for the register-level routines) ---- */
 #define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out */
