@@ -110,6 +110,7 @@ extern "C" int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb
     l->Vh = (uint32_t)(V / 2);
     l->S = (l->Vh + 63u) & ~63u;
     l->esize = prec == QCB_F64 ? 8 : 4;
+    for (int d = 1; d < 4; ++d) l->tile[d] = default_tile(dims[d]);
     int rc = lattice_alloc(l);
     if (rc != QCB_OK) {
         lattice_free(l);
@@ -135,6 +136,16 @@ extern "C" int qcb_lattice_destroy(qcb_lattice* lat) {
     lattice_free(lat);
     delete lat;
     return QCB_OK;
+}
+
+extern "C" int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt) {
+    if (!lat) return QCB_EINVAL;
+    const int t[4] = {0, ty, tz, tt};
+    for (int d = 1; d < 4; ++d)
+        if (t[d] < 1 || lat->L[d] % t[d])
+            return qcb_set_error(lat->ctx, QCB_EINVAL, "tile extent %d does not divide %d", t[d], lat->L[d]);
+    for (int d = 1; d < 4; ++d) lat->tile[d] = t[d];
+    return upload_desc(lat);
 }
 
 int qcb_lattice_dims_internal(qcb_lattice* lat, int dims[4]) {

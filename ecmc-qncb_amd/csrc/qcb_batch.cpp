@@ -1,0 +1,386 @@
+// qcb_batch.cpp -- many independent lattices (uneven sizes, own beta and
+// Philox key each) advanced by one launch per (mu, parity) half-sweep, per RK3
+// stage and per reduction; a tuned launch schedule; hipGraph replay of sweeps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qcb.h"
+#include "qcb_host.hpp"
+#include "qcb_internal.hpp"
+
+using namespace qcb;
+
+int enqueue_flow_step(qcb_lattice* lat, double eps);
+
+struct qcb_batch {
+    qcb_ctx* ctx = nullptr;
+    int prec = QCB_F64;
+    int n = 0;
+    std::vector<qcb_lattice*> lats;
+    void* d_desc = nullptr;
+    uint32_t* d_blk[3] = {nullptr, nullptr, nullptr};
+    uint32_t nb[3] = {0, 0, 0};
+    double* d_partial = nullptr;     // 2 * nb[kFamSite]
+    double* d_meas = nullptr;        // measurement buffer (grown on demand)
+    size_t meas_cap = 0;
+    uint32_t* d_sweep = nullptr;     // device sweep counter for graph replay
+    unsigned long long* d_counter = nullptr;
+    uint32_t gstep = 0;
+    // launch schedule (autotuned)
+    int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};   // half-sweep order, hs = 2*mu + parity
+    uint32_t hb_grid = 0;                       // 0: one block per tile; else persistent grid
+    // captured sweeps
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_sweeps = 0;
+};
+
+namespace {
+
+int batch_upload(qcb_batch* b) {
+    qcb_ctx* c = b->ctx;
+    if (b->prec == QCB_F64) {
+        std::vector<LatDev<double>> d(b->n);
+        for (int i = 0; i < b->n; ++i) d[i] = make_desc<double>(b->lats[i]);
+        QCB_HIP(c, hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(d[0]), hipMemcpyHostToDevice, c->stream));
+        QCB_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<LatDev<float>> d(b->n);
+        for (int i = 0; i < b->n; ++i) d[i] = make_desc<float>(b->lats[i]);
+        QCB_HIP(c, hipMemcpyAsync(b->d_desc, d.data(), d.size() * sizeof(d[0]), hipMemcpyHostToDevice, c->stream));
+        QCB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return QCB_OK;
+}
+
+Launch batch_launch(const qcb_batch* b, int fam) {
+    Launch g;
+    g.lats = b->d_desc;
+    g.blk = b->d_blk[fam];
+    g.n_lat = b->n;
+    g.nblocks = b->nb[fam];
+    g.stream = b->ctx->stream;
+    return g;
+}
+
+void batch_free(qcb_batch* b) {
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (qcb_lattice* l : b->lats)
+        if (l) qcb_lattice_destroy(l);
+    (void)hipFree(b->d_desc);
+    for (int f = 0; f < 3; ++f) (void)hipFree(b->d_blk[f]);
+    (void)hipFree(b->d_partial);
+    (void)hipFree(b->d_meas);
+    (void)hipFree(b->d_sweep);
+    (void)hipFree(b->d_counter);
+}
+
+int ensure_meas(qcb_batch* b, size_t n) {
+    if (n <= b->meas_cap) return QCB_OK;
+    (void)hipFree(b->d_meas);
+    b->d_meas = nullptr;
+    QCB_HIP(b->ctx, hipMalloc((void**)&b->d_meas, n * sizeof(double)));
+    b->meas_cap = n;
+    return QCB_OK;
+}
+
+int enqueue_batch_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx, const uint32_t* base) {
+    Launch g = batch_launch(b, kFamHalf);
+    for (int s = 0; s < n_sweeps; ++s)
+        for (int k = 0; k < 8; ++k) {
+            const int mu = b->order[k] >> 1, p = b->order[k] & 1;
+            if (b->prec == QCB_F64)
+                launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, b->d_counter,
+                                       base, b->hb_grid);
+            else
+                launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, b->d_counter,
+                                      base, b->hb_grid);
+        }
+    QCB_HIP(b->ctx, hipGetLastError());
+    return QCB_OK;
+}
+
+}  // namespace
+
+extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, const double* betas,
+                                const uint64_t* seeds, qcb_batch** out) {
+    if (!ctx || n < 1 || !dims || !betas || !seeds || !out) return QCB_EINVAL;
+    *out = nullptr;
+    qcb_batch* b = new qcb_batch();
+    b->ctx = ctx;
+    b->prec = prec;
+    b->n = n;
+    b->lats.assign(n, nullptr);
+    int rc;
+    for (int i = 0; i < n; ++i) {
+        rc = qcb_lattice_create(ctx, dims + 4 * i, prec, &b->lats[i]);
+        if (rc) {
+            batch_free(b);
+            delete b;
+            return rc;
+        }
+        b->lats[i]->beta = betas[i];
+        b->lats[i]->seed = seeds[i];
+        if ((rc = upload_desc(b->lats[i]))) {
+            batch_free(b);
+            delete b;
+            return rc;
+        }
+    }
+    QCB_HIP(ctx, hipMalloc(&b->d_desc, (size_t)n * sizeof(LatDev<double>)));
+    for (int f = 0; f < 3; ++f) {
+        std::vector<uint32_t> pre(n + 1, 0);
+        for (int i = 0; i < n; ++i) pre[i + 1] = pre[i] + fam_blocks(b->lats[i]->Vh, f);
+        b->nb[f] = pre[n];
+        QCB_HIP(ctx, hipMalloc((void**)&b->d_blk[f], pre.size() * sizeof(uint32_t)));
+        QCB_HIP(ctx, hipMemcpy(b->d_blk[f], pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    }
+    QCB_HIP(ctx, hipMalloc((void**)&b->d_partial, 2 * sizeof(double) * b->nb[kFamSite]));
+    QCB_HIP(ctx, hipMalloc((void**)&b->d_sweep, sizeof(uint32_t)));
+    QCB_HIP(ctx, hipMemset(b->d_sweep, 0, sizeof(uint32_t)));
+    QCB_HIP(ctx, hipMalloc((void**)&b->d_counter, sizeof(unsigned long long)));
+    QCB_HIP(ctx, hipMemset(b->d_counter, 0, sizeof(unsigned long long)));
+    rc = batch_upload(b);
+    if (rc) {
+        batch_free(b);
+        delete b;
+        return rc;
+    }
+    ctx->n_live++;
+    *out = b;
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_destroy(qcb_batch* b) {
+    if (!b) return QCB_EINVAL;
+    (void)hipStreamSynchronize(b->ctx->stream);
+    b->ctx->n_live--;
+    batch_free(b);
+    delete b;
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_lattice(qcb_batch* b, int i, qcb_lattice** out) {
+    if (!b || !out || i < 0 || i >= b->n) return QCB_EINVAL;
+    *out = b->lats[i];
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_hot(qcb_batch* b) {
+    if (!b) return QCB_EINVAL;
+    Launch g = batch_launch(b, kFamLink);
+    if (b->prec == QCB_F64) launch_hot<double>(g); else launch_hot<float>(g);
+    QCB_HIP(b->ctx, hipGetLastError());
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_hb_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx) {
+    if (!b || n_hits < 1 || n_hits > 4095 || n_sweeps < 0) return QCB_EINVAL;
+    return enqueue_batch_sweeps(b, n_hits, n_sweeps, sweep_idx, nullptr);
+}
+
+extern "C" int qcb_batch_reunitarize(qcb_batch* b) {
+    if (!b) return QCB_EINVAL;
+    Launch g = batch_launch(b, kFamLink);
+    if (b->prec == QCB_F64) launch_reunit<double>(g); else launch_reunit<float>(g);
+    QCB_HIP(b->ctx, hipGetLastError());
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_plaquette(qcb_batch* b, double* out) {
+    if (!b || !out) return QCB_EINVAL;
+    int rc = ensure_meas(b, (size_t)b->n);
+    if (rc) return rc;
+    Launch g = batch_launch(b, kFamSite);
+    if (b->prec == QCB_F64) {
+        launch_plaquette<double>(g, b->d_partial);
+        launch_finalize_plaq<double>(g, b->d_partial, b->d_meas, 1);
+    } else {
+        launch_plaquette<float>(g, b->d_partial);
+        launch_finalize_plaq<float>(g, b->d_partial, b->d_meas, 1);
+    }
+    QCB_HIP(b->ctx, hipGetLastError());
+    QCB_HIP(b->ctx, hipMemcpyAsync(out, b->d_meas, b->n * sizeof(double), hipMemcpyDeviceToHost, b->ctx->stream));
+    QCB_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    return QCB_OK;
+}
+
+// Flow every lattice of the batch from its links: measure (Q, E/V) at
+// t = 0, eps, ..., n_steps*eps (measure-then-step, plus the final state).
+// tQE[(i*(n_steps+1) + s)*3 + {0,1,2}] = (t, Q, E_mean) of lattice i.
+extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE) {
+    if (!b || n_steps < 0 || !tQE) return QCB_EINVAL;
+    qcb_ctx* c = b->ctx;
+    const int nm = n_steps + 1;
+    int rc = ensure_meas(b, (size_t)b->n * nm * 2);
+    if (rc) return rc;
+    for (qcb_lattice* l : b->lats) {
+        if ((rc = ensure_flow_buffers(l))) return rc;
+        l->flow_start = b->gstep;
+        l->flow_active = true;
+    }
+    if ((rc = batch_upload(b))) return rc;
+    Launch gl = batch_launch(b, kFamLink), gs = batch_launch(b, kFamSite);
+    double* pq = b->d_partial;
+    double* pe = b->d_partial + b->nb[kFamSite];
+    // copy links into the flow fields of every lattice: one launch per lattice
+    for (qcb_lattice* l : b->lats) {
+        Launch one = lat_launch(l, kFamLink);
+        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
+        else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
+    }
+    std::vector<double> tv(nm);
+    double t = 0.0;
+    for (int s = 0; s < nm; ++s) {
+        tv[s] = t;
+        if (b->prec == QCB_F64) {
+            launch_clover<double>(gs, 1, b->gstep, pq, pe);
+            launch_finalize_qe<double>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
+        } else {
+            launch_clover<float>(gs, 1, b->gstep, pq, pe);
+            launch_finalize_qe<float>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
+        }
+        if (s == n_steps) break;
+        for (int st = 1; st <= 3; ++st) {
+            if (b->prec == QCB_F64) launch_flow_stage<double>(gl, st, b->gstep, eps);
+            else launch_flow_stage<float>(gl, st, b->gstep, eps);
+        }
+        b->gstep++;
+        t += eps;
+    }
+    QCB_HIP(c, hipGetLastError());
+    std::vector<double> h((size_t)b->n * nm * 2);
+    QCB_HIP(c, hipMemcpyAsync(h.data(), b->d_meas, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    QCB_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < b->n; ++i) {
+        b->lats[i]->gstep = b->gstep;
+        if ((rc = upload_desc(b->lats[i]))) return rc;   // single-lattice calls see the flow state
+        for (int s = 0; s < nm; ++s) {
+            tQE[((size_t)i * nm + s) * 3] = tv[s];
+            tQE[((size_t)i * nm + s) * 3 + 1] = h[(size_t)i * nm * 2 + 2 * s];
+            tQE[((size_t)i * nm + s) * 3 + 2] = h[(size_t)i * nm * 2 + 2 * s + 1];
+        }
+    }
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_set_schedule(qcb_batch* b, const int order[8], uint32_t hb_grid) {
+    if (!b || !order || hb_grid % 8) return QCB_EINVAL;
+    int seen = 0;
+    for (int k = 0; k < 8; ++k) {
+        if (order[k] < 0 || order[k] > 7 || (seen >> order[k]) & 1) return QCB_EINVAL;
+        seen |= 1 << order[k];
+    }
+    std::memcpy(b->order, order, sizeof b->order);
+    b->hb_grid = hb_grid;
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_get_schedule(qcb_batch* b, int order[8], uint32_t* hb_grid) {
+    if (!b || !order || !hb_grid) return QCB_EINVAL;
+    std::memcpy(order, b->order, sizeof b->order);
+    *hb_grid = b->hb_grid;
+    return QCB_OK;
+}
+
+// Times candidate schedules (half-sweep order x persistent grid) on a scratch
+// copy of every lattice's links and keeps the fastest.  The links themselves
+// are not modified.  report (optional) receives one line per candidate.
+extern "C" int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len) {
+    if (!b || n_hits < 1 || reps < 1) return QCB_EINVAL;
+    qcb_ctx* c = b->ctx;
+    int rc;
+    // tune on scratch copies: swap U with F[0] after copying
+    for (qcb_lattice* l : b->lats) {
+        if ((rc = ensure_flow_buffers(l))) return rc;
+        Launch one = lat_launch(l, kFamLink);
+        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
+        else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
+        std::swap(l->U, l->F[0]);
+        l->flow_active = false;
+    }
+    if ((rc = batch_upload(b))) return rc;
+    int dev = 0, ncu = 0;
+    QCB_HIP(c, hipGetDevice(&dev));
+    QCB_HIP(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int orders[2][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {0, 2, 4, 6, 1, 3, 5, 7}};
+    std::vector<uint32_t> grids = {0};
+    for (int per : {2, 4, 8}) {
+        uint32_t g = (uint32_t)(ncu * per) & ~7u;
+        if (g && g < b->nb[kFamHalf]) grids.push_back(g);
+    }
+    std::string rep;
+    double best = 1e300;
+    int best_o = 0;
+    uint32_t best_g = 0;
+    for (int o = 0; o < 2; ++o)
+        for (uint32_t g : grids) {
+            std::memcpy(b->order, orders[o], sizeof b->order);
+            b->hb_grid = g;
+            if ((rc = enqueue_batch_sweeps(b, n_hits, 1, 0x7fff0000u, nullptr))) return rc;   // warm-up
+            QCB_HIP(c, hipEventRecord(c->ev0, c->stream));
+            if ((rc = enqueue_batch_sweeps(b, n_hits, reps, 0x7fff0001u, nullptr))) return rc;
+            QCB_HIP(c, hipEventRecord(c->ev1, c->stream));
+            QCB_HIP(c, hipEventSynchronize(c->ev1));
+            float ms = 0.f;
+            QCB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            char line[160];
+            std::snprintf(line, sizeof line, "order=%s grid=%u ms_per_sweep=%.4f\n", o ? "parity-major" : "mu-major", g,
+                          ms / reps);
+            rep += line;
+            if (ms < best) {
+                best = ms;
+                best_o = o;
+                best_g = g;
+            }
+        }
+    std::memcpy(b->order, orders[best_o], sizeof b->order);
+    b->hb_grid = best_g;
+    for (qcb_lattice* l : b->lats) std::swap(l->U, l->F[0]);
+    if ((rc = batch_upload(b))) return rc;
+    if (report && report_len) {
+        std::snprintf(report, report_len, "%s", rep.c_str());
+    }
+    return QCB_OK;
+}
+
+// Captures sweeps_per_graph sweeps (8 half-sweep launches each, with the
+// Philox sweep index read from a device counter) plus the counter update into
+// a hipGraph; qcb_batch_graph_launch replays it.  Sweep indices continue from
+// first_sweep across replays.
+extern "C" int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_graph, uint32_t first_sweep) {
+    if (!b || n_hits < 1 || sweeps_per_graph < 1) return QCB_EINVAL;
+    qcb_ctx* c = b->ctx;
+    if (b->graph_exec) {
+        QCB_HIP(c, hipGraphExecDestroy(b->graph_exec));
+        b->graph_exec = nullptr;
+    }
+    if (b->graph) {
+        QCB_HIP(c, hipGraphDestroy(b->graph));
+        b->graph = nullptr;
+    }
+    QCB_HIP(c, hipMemcpyAsync(b->d_sweep, &first_sweep, sizeof first_sweep, hipMemcpyHostToDevice, c->stream));
+    QCB_HIP(c, hipStreamSynchronize(c->stream));
+    QCB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_batch_sweeps(b, n_hits, sweeps_per_graph, 0, b->d_sweep);
+    launch_bump(b->d_sweep, (uint32_t)sweeps_per_graph, c->stream);
+    hipError_t e = hipStreamEndCapture(c->stream, &b->graph);
+    if (rc) return rc;
+    if (e != hipSuccess) return qcb_set_error(c, QCB_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    QCB_HIP(c, hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
+    b->graph_sweeps = sweeps_per_graph;
+    return QCB_OK;
+}
+
+extern "C" int qcb_batch_graph_launch(qcb_batch* b, int n_replays) {
+    if (!b || n_replays < 0) return QCB_EINVAL;
+    if (!b->graph_exec) return qcb_set_error(b->ctx, QCB_ESTATE, "no captured graph");
+    for (int r = 0; r < n_replays; ++r) QCB_HIP(b->ctx, hipGraphLaunch(b->graph_exec, b->ctx->stream));
+    return QCB_OK;
+}

@@ -37,6 +37,7 @@ struct qcb_lattice {
     uint64_t seed = 0;
     uint32_t gstep = 0, flow_start = 0;
     bool flow_active = false;   // flow buffers hold a GradientFlow state (qcb_flow_copy)
+    int tile[4] = {0, 1, 1, 1};  // work-order tile extents (tile[0] unused: whole x rows)
 };
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
@@ -53,8 +54,17 @@ namespace qcb {
 
 inline uint64_t field_bytes(const qcb_lattice* l) { return (uint64_t)8 * 18 * l->S * (uint64_t)l->esize; }
 
-inline uint32_t fam_units(const qcb_lattice* l, int fam) {
-    return fam == kFamHalf ? l->Vh : fam == kFamSite ? 2 * l->Vh : 8 * l->Vh;
+// Default work-order tile extent along one direction: largest divisor <= 4.
+inline int default_tile(int L) {
+    for (int t = 4; t > 1; --t)
+        if (L % t == 0) return t;
+    return 1;
+}
+
+// Blocks one lattice contributes to a launch of the given kernel family.
+inline uint32_t fam_blocks(uint32_t Vh, int fam) {
+    const uint32_t per = fam == kFamHalf ? kHalfSites : fam == kFamSite ? kSiteSites : kLinkSites;
+    return (Vh + per - 1) / per;
 }
 
 template <class T>
@@ -70,6 +80,13 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
     d.key0 = (uint32_t)l->seed;
     d.key1 = (uint32_t)(l->seed >> 32);
     d.flow_start = l->flow_start;
+    d.TL[0] = l->L[0];
+    d.NT[0] = 1;
+    for (int k = 1; k < 4; ++k) {
+        d.TL[k] = l->tile[k];
+        d.NT[k] = l->L[k] / l->tile[k];
+    }
+    d.tvol = (uint32_t)(l->L[0] / 2) * l->tile[1] * l->tile[2] * l->tile[3];
     return d;
 }
 
@@ -102,7 +119,7 @@ inline int lattice_alloc(qcb_lattice* l) {
         return qcb_set_error(c, QCB_ENOMEM, "cannot allocate %llu bytes of links", (unsigned long long)field_bytes(l));
     QCB_HIP(c, hipMalloc(&l->d_desc, sizeof(LatDev<double>)));
     for (int f = 0; f < 3; ++f) {
-        l->nb[f] = (fam_units(l, f) + kBlock - 1) / kBlock;
+        l->nb[f] = fam_blocks(l->Vh, f);
         uint32_t h[2] = {0, l->nb[f]};
         QCB_HIP(c, hipMalloc((void**)&l->d_blk[f], sizeof h));
         QCB_HIP(c, hipMemcpy(l->d_blk[f], h, sizeof h, hipMemcpyHostToDevice));

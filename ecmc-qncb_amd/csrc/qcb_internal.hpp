@@ -30,8 +30,12 @@ template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst)
 template <class T> void launch_plaquette(const Launch& g, double* partial);
 template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
 template <class T> void launch_staples(const Launch& g, T* out_field);
+// sweep_base: optional device counter added to sweep_idx (graph replay);
+// grid: 0 = one block per tile, else a persistent grid (a multiple of 8)
 template <class T> void launch_hb_half(const Launch& g, int mu, int parity, int n_hits, uint32_t sweep_idx,
-                                       int mode, const double* inj, int K, unsigned long long* exhausted);
+                                       int mode, const double* inj, int K, unsigned long long* exhausted,
+                                       const uint32_t* sweep_base = nullptr, uint32_t grid = 0);
+void launch_bump(uint32_t* counter, uint32_t n, hipStream_t st);
 template <class T> void launch_reunit(const Launch& g);
 template <class T> void launch_translate(const Launch& g, const int s[4], T* dst_field);
 // One RK3 stage on every lattice of the batch.  Field roles follow the

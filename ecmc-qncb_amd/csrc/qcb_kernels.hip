@@ -60,9 +60,9 @@ __global__ __launch_bounds__(kBlock) void k_cold(const LatDev<T>* __restrict__ l
                                                  int n_lat, uint32_t nblocks) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     M3<T> m;
     m3_identity(m);
     st_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i, m);
@@ -75,9 +75,9 @@ __global__ __launch_bounds__(kBlock) void k_hot(const LatDev<T>* __restrict__ la
                                                 int n_lat, uint32_t nblocks) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     Coords c = cb_coords(L, i, p);
     uint32_t lex = (uint32_t)c.x + (uint32_t)L.L[0] * ((uint32_t)c.y + (uint32_t)L.L[1] * ((uint32_t)c.z + (uint32_t)L.L[2] * (uint32_t)c.t));
@@ -150,9 +150,9 @@ __global__ __launch_bounds__(kBlock) void k_aos_to_soa(const LatDev<T>* __restri
                                                        T* __restrict__ dst) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     Coords c = cb_coords(L, i, p);
     size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
@@ -169,9 +169,9 @@ __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const LatDev<T>* __restri
                                                        double* __restrict__ aos) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     Coords c = cb_coords(L, i, p);
     size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
@@ -187,9 +187,9 @@ __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restri
                                                        T* __restrict__ dst) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     st_link(dst, L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(src, L.S, (int)(hs >> 1), (int)(hs & 1), i));
 }
 
@@ -200,9 +200,9 @@ __global__ __launch_bounds__(kBlock) void k_translate(const LatDev<T>* __restric
                                                       T* __restrict__ dst) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     Coords c = cb_coords(L, i, p);
     int x = c.x - s0; x += (x < 0) ? L.L[0] : 0;
@@ -218,9 +218,9 @@ __global__ __launch_bounds__(kBlock) void k_reunit(const LatDev<T>* __restrict__
                                                    int n_lat, uint32_t nblocks) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     M3<T> m = ld_link(L.U, L.S, mu, p, i);
     M3<double> d;
@@ -242,12 +242,12 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
     __shared__ double lds[kBlock / 64];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
+    const uint32_t qs = bc.local * kSiteSites + (threadIdx.x & (kSiteSites - 1));
+    const int p = (int)(threadIdx.x / kSiteSites);
     double sum = 0.0;
-    if (j < 2u * L.Vh) {
-        int p = j >= L.Vh;
-        uint32_t i = j - (p ? L.Vh : 0u);
-        Coords c = cb_coords(L, i, p);
+    if (qs < L.Vh) {
+        Coords c = tiled_coords(L, qs, p);
+        const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
         const T* f = L.U;
 #pragma unroll
         for (int mu = 0; mu < 3; ++mu) {
@@ -329,12 +329,12 @@ __global__ __launch_bounds__(kBlock) void k_clover(const LatDev<T>* __restrict__
     __shared__ double lds[kBlock / 64];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
+    const uint32_t qs = bc.local * kSiteSites + (threadIdx.x & (kSiteSites - 1));
+    const int p = (int)(threadIdx.x / kSiteSites);
     double q = 0.0, e = 0.0;
-    if (j < 2u * L.Vh) {
-        int p = j >= L.Vh;
-        uint32_t i = j - (p ? L.Vh : 0u);
-        Coords c = cb_coords(L, i, p);
+    if (qs < L.Vh) {
+        Coords c = tiled_coords(L, qs, p);
+        const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
         const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
         double tq = 0.0;
         {
@@ -409,11 +409,12 @@ __global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict_
                                                     int n_lat, uint32_t nblocks, T* __restrict__ out) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t q = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (q >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
-    Coords c = cb_coords(L, i, p);
+    Coords c = tiled_coords(L, q, p);
+    const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     switch (mu) {
         case 0: staple_one<T, 0>(L, out, c, p, i); break;
         case 1: staple_one<T, 1>(L, out, c, p, i); break;
@@ -511,12 +512,10 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
             const uint32_t cz = hb_ctr_z(hs, h, g);
             T a0 = T(0);
             for (uint32_t a = 0; a < kPhiloxMaxAttempts; ++a) {
-                U4 o0 = philox4x32_10(U4{cb, sweep, cz, 2 * a}, key0, key1);
-                U4 o1 = philox4x32_10(U4{cb, sweep, cz, 2 * a + 1}, key0, key1);
-                if (kp_try<T>(alpha, (T)u53(o0.x, o0.y), (T)u53(o0.z, o0.w), (T)u53(o1.x, o1.y), (T)u53(o1.z, o1.w),
-                              a0)) {
+                U4 o = philox4x32_10(U4{cb, sweep, cz, a}, key0, key1);
+                if (kp_try<T>(alpha, (T)u32d(o.x), (T)u32d(o.y), (T)u32d(o.z), (T)u32d(o.w), a0)) {
                     U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
-                    kp_sphere<T>(a0, (T)u53(os.x, os.y), (T)u53(os.z, os.w), x);
+                    kp_sphere<T>(a0, (T)u32d(os.x), (T)u32d(os.y), x);
                     ok = true;
                     break;
                 }
@@ -552,24 +551,44 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
 
 // One (mu, parity) half-sweep: each thread owns one link U_mu(x), computes its
 // staple once and applies n_hits CM hits in registers.
+// Occupancy floor of the heavy kernels: with no floor the scheduler hoists
+// every neighbour load and the double kernels fall to one wave per SIMD.
+template <class T> struct MinWaves;
+template <> struct MinWaves<float> { static constexpr int hb = 3, flow = 4; };
+template <> struct MinWaves<double> { static constexpr int hb = 2, flow = 2; };
+
 template <class T, int MU, int MODE>
-__global__ __launch_bounds__(kBlock) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::hb))) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                     int n_lat, uint32_t nblocks, int p, int n_hits, uint32_t sweep,
+                                                    const uint32_t* __restrict__ sweep_base,
                                                     const double* __restrict__ inj, int K,
                                                     unsigned long long* __restrict__ exhausted) {
-    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
-    const LatDev<T>& L = *bc.L;
-    uint32_t i = bc.local * kBlock + threadIdx.x;
-    if (i >= L.Vh) return;
-    Coords c = cb_coords(L, i, p);
-    M3<T> A = staple<T, MU>(L, L.U, c, p, i);
-    M3<T> U = ld_link(L.U, L.S, MU, p, i);
-    const T beta = (T)L.beta;
-    int ex = 0;
-    for (int h = 0; h < n_hits; ++h)
-        ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sweep, L.key0, L.key1, inj, n_hits, K, L.Vh);
-    st_link(L.U, L.S, MU, p, i, U);
-    if (ex) atomicAdd(exhausted, (unsigned long long)ex);
+    // sweep_base (device counter, advanced by k_bump) lets a captured graph of
+    // sweeps be replayed with fresh Philox counters
+    if (sweep_base) sweep += *sweep_base;
+    // persistent grids stride over the tiles; gridDim.x is a multiple of 8 so
+    // a block keeps to its XCD's contiguous tile range
+    for (uint32_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint32_t tile = xcd_tile(b, nblocks);
+        const int l = find_lat(blk, n_lat, tile);
+        const LatDev<T>& L = lats[l];
+        const uint32_t q = (tile - blk[l]) * kHalfSites + threadIdx.x;
+        if (q >= L.Vh) continue;
+        Coords c = tiled_coords(L, q, p);
+        const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
+        M3<T> A = staple<T, MU>(L, L.U, c, p, i);
+        M3<T> U = ld_link(L.U, L.S, MU, p, i);
+        const T beta = (T)L.beta;
+        int ex = 0;
+        for (int h = 0; h < n_hits; ++h)
+            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sweep, L.key0, L.key1, inj, n_hits, K, L.Vh);
+        st_link(L.U, L.S, MU, p, i, U);
+        if (ex) atomicAdd(exhausted, (unsigned long long)ex);
+    }
+}
+
+__global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *counter += n;
 }
 
 // --------------------------------------------------------------- flow ------
@@ -610,19 +629,20 @@ __device__ __forceinline__ void flow_link(const LatDev<T>& L, const T* __restric
 }
 
 template <class T, int STAGE>
-__global__ __launch_bounds__(kBlock) void k_flow_stage(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::flow))) void k_flow_stage(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, uint32_t gstep, T eps) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    uint32_t j = bc.local * kBlock + threadIdx.x;
-    if (j >= 8u * L.Vh) return;
-    uint32_t hs = j / L.Vh, i = j - hs * L.Vh;
+    const uint32_t q = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (q >= L.Vh) return;
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     const int cur = (int)((gstep - L.flow_start) & 1u);
     const T* win = L.F[(STAGE == 2) ? 1 - cur : cur];
     T* wout = L.F[(STAGE == 2) ? cur : 1 - cur];
     T* zs = L.F[2];
-    Coords c = cb_coords(L, i, p);
+    Coords c = tiled_coords(L, q, p);
+    const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     switch (mu) {
         case 0: flow_link<T, 0, STAGE>(L, win, wout, zs, c, p, i, eps); break;
         case 1: flow_link<T, 1, STAGE>(L, win, wout, zs, c, p, i, eps); break;
@@ -728,25 +748,30 @@ void launch_translate(const Launch& g, const int s[4], T* dst) {
 }
 
 template <class T, int MU>
-static void hb_mu(const Launch& g, int p, int n_hits, uint32_t sweep, int mode, const double* inj, int K,
-                  unsigned long long* ex) {
+static void hb_mu(const Launch& g, int p, int n_hits, uint32_t sweep, const uint32_t* base, int mode, const double* inj,
+                  int K, unsigned long long* ex, uint32_t grid) {
     if (mode == kHbInjected)
-        hipLaunchKernelGGL((k_hb_half<T, MU, kHbInjected>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
-                           n_hits, sweep, inj, K, ex);
+        hipLaunchKernelGGL((k_hb_half<T, MU, kHbInjected>), dim3(grid), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
+                           n_hits, sweep, base, inj, K, ex);
     else
-        hipLaunchKernelGGL((k_hb_half<T, MU, kHbPhilox>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
-                           n_hits, sweep, inj, K, ex);
+        hipLaunchKernelGGL((k_hb_half<T, MU, kHbPhilox>), dim3(grid), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
+                           n_hits, sweep, base, inj, K, ex);
 }
 
 template <class T>
 void launch_hb_half(const Launch& g, int mu, int p, int n_hits, uint32_t sweep, int mode, const double* inj, int K,
-                    unsigned long long* ex) {
+                    unsigned long long* ex, const uint32_t* sweep_base, uint32_t grid) {
+    if (grid == 0 || grid > g.nblocks) grid = g.nblocks;
     switch (mu) {
-        case 0: hb_mu<T, 0>(g, p, n_hits, sweep, mode, inj, K, ex); break;
-        case 1: hb_mu<T, 1>(g, p, n_hits, sweep, mode, inj, K, ex); break;
-        case 2: hb_mu<T, 2>(g, p, n_hits, sweep, mode, inj, K, ex); break;
-        default: hb_mu<T, 3>(g, p, n_hits, sweep, mode, inj, K, ex); break;
+        case 0: hb_mu<T, 0>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
+        case 1: hb_mu<T, 1>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
+        case 2: hb_mu<T, 2>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
+        default: hb_mu<T, 3>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
     }
+}
+
+void launch_bump(uint32_t* counter, uint32_t n, hipStream_t st) {
+    hipLaunchKernelGGL(k_bump, dim3(1), dim3(64), 0, st, counter, n);
 }
 
 template <class T>
@@ -787,7 +812,7 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_reunit<T>(const Launch&);                                                            \
     template void launch_translate<T>(const Launch&, const int*, T*);                                         \
     template void launch_hb_half<T>(const Launch&, int, int, int, uint32_t, int, const double*, int,           \
-                                    unsigned long long*);                                                     \
+                                    unsigned long long*, const uint32_t*, uint32_t);                          \
     template void launch_flow_stage<T>(const Launch&, int, uint32_t, double);                                 \
     template void launch_math_probe<T>(int, int, const double*, double, int, double*, hipStream_t);           \
     template void launch_finalize_plaq<T>(const Launch&, const double*, double*, int);                        \

@@ -1,15 +1,22 @@
 // qcb_layout.hpp -- device link storage and periodic even/odd indexing.
 //
-// HBM layout (one lattice): structure-of-arrays, even/odd split,
-//   field[((mu*2 + parity)*18 + comp)*S + cb]
-// comp = 2*(3*i + j) + {re,im}, cb = lex/2 with lex = x + Lx*(y + Ly*(z + Lz*t)),
-// parity = (x+y+z+t) & 1 and S = Vh rounded up to 64 (each component array
-// starts 256-byte aligned in float).  A 64-lane wavefront reading one
-// component of 64 consecutive cb sites issues one fully coalesced 256/512-byte
-// load.  The reference keeps an AoS field with a one-site halo and a
-// neighbour table (src/gauge/GaugeField.h:27, src/geometry/GeometryCB.cpp:10-31);
-// here neighbours are closed-form periodic index arithmetic, so the halo
-// exchange of src/mpi/HalosExchange.cpp has no counterpart.
+// HBM layout (one lattice): structure-of-arrays over complex entries,
+// even/odd split,
+//   field[((mu*2 + parity)*9 + k)*S + cb]   (one complex = 2 reals)
+// k = 3*i + j, cb = lex/2 with lex = x + Lx*(y + Ly*(z + Lz*t)),
+// parity = (x+y+z+t) & 1 and S = Vh rounded up to 64.  A 64-lane wavefront
+// reading entry k of 64 consecutive cb sites issues one fully coalesced
+// 512-byte (float) / 1 KiB (double) load of 8/16-byte elements.  The
+// reference keeps an AoS field with a one-site halo and a neighbour table
+// (src/gauge/GaugeField.h:27, src/geometry/GeometryCB.cpp:10-31); here
+// neighbours are closed-form periodic index arithmetic, so the halo exchange
+// of src/mpi/HalosExchange.cpp has no counterpart.
+//
+// Work order: kernels that read neighbours visit sites in a 4D-tiled order
+// (tiles of Lx x T1 x T2 x T3 sites, tiles y-fastest), so that the +-t and +-z
+// neighbours of most sites were fetched by the same or a recent block and
+// are still in L2.  The tiling changes only which thread handles which site,
+// never the cb index a site's data or random numbers live at.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -34,8 +41,16 @@ struct LatDev {
     double beta;
     uint32_t key0, key1;  // Philox key (seed)
     uint32_t flow_start;  // global flow-step index at which this lattice's flow began
-    uint32_t pad;
+    int TL[4];            // tile extents in sites (TL[0] = Lx: whole x rows)
+    int NT[4];            // tiles per direction (NT[0] = 1)
+    uint32_t tvol;        // cb sites per tile = (Lx/2) * T1 * T2 * T3
 };
+
+// Threads per block of each kernel family and the work item of a thread:
+//   half-sweep: 256 cb sites of one (mu, parity)
+//   site kernels: 128 cb sites x 2 parities
+//   link kernels: 32 cb sites x 8 (mu, parity) classes
+constexpr uint32_t kHalfSites = 256, kSiteSites = 128, kLinkSites = 32;
 
 struct Coords {
     int x, y, z, t;
@@ -60,6 +75,31 @@ __device__ __forceinline__ uint32_t cb_index(const LatDev<T>& L, int x, int y, i
     return (uint32_t)(x >> 1) + (uint32_t)L.Lxh * ((uint32_t)y + (uint32_t)L.L[1] * ((uint32_t)z + (uint32_t)L.L[2] * (uint32_t)t));
 }
 
+// Coordinates of the q-th site of parity p in the tiled work order.
+template <class T>
+__device__ __forceinline__ Coords tiled_coords(const LatDev<T>& L, uint32_t q, int p) {
+    uint32_t tau = q / L.tvol;
+    uint32_t r = q - tau * L.tvol;
+    uint32_t r1 = r / (uint32_t)L.Lxh;
+    int xh = (int)(r - r1 * (uint32_t)L.Lxh);
+    uint32_t r2 = r1 / (uint32_t)L.TL[1];
+    int yl = (int)(r1 - r2 * (uint32_t)L.TL[1]);
+    uint32_t r3 = r2 / (uint32_t)L.TL[2];
+    int zl = (int)(r2 - r3 * (uint32_t)L.TL[2]);
+    int tl = (int)r3;
+    uint32_t u1 = tau / (uint32_t)L.NT[1];
+    int ty = (int)(tau - u1 * (uint32_t)L.NT[1]);
+    uint32_t u2 = u1 / (uint32_t)L.NT[2];
+    int tz = (int)(u1 - u2 * (uint32_t)L.NT[2]);
+    int tt = (int)u2;
+    Coords c;
+    c.y = ty * L.TL[1] + yl;
+    c.z = tz * L.TL[2] + zl;
+    c.t = tt * L.TL[3] + tl;
+    c.x = 2 * xh + ((c.y + c.z + c.t + p) & 1);
+    return c;
+}
+
 __device__ __forceinline__ int wrap_up(int v, int n) { return v + 1 == n ? 0 : v + 1; }
 __device__ __forceinline__ int wrap_dn(int v, int n) { return v == 0 ? n - 1 : v - 1; }
 
@@ -78,24 +118,28 @@ __device__ __forceinline__ uint32_t nb_cb(const LatDev<T>& L, Coords c, int d1, 
 }
 
 template <class T>
-__device__ __forceinline__ const T* comp_base(const T* f, uint32_t S, int mu, int p) {
-    return f + (size_t)((mu * 2 + p) * 18) * S;
-}
+struct alignas(2 * sizeof(T)) Cplx {
+    T re, im;
+};
 
 template <class T>
 __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, uint32_t i) {
-    const T* b = comp_base(f, S, mu, p) + i;
+    const Cplx<T>* b = reinterpret_cast<const Cplx<T>*>(f) + (size_t)((mu * 2 + p) * 9) * S + i;
     M3<T> m;
 #pragma unroll
-    for (int k = 0; k < 18; ++k) m.v[k] = b[(size_t)k * S];
+    for (int k = 0; k < 9; ++k) {
+        Cplx<T> c = b[(size_t)k * S];
+        m.v[2 * k] = c.re;
+        m.v[2 * k + 1] = c.im;
+    }
     return m;
 }
 
 template <class T>
 __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_t i, const M3<T>& m) {
-    T* b = f + (size_t)((mu * 2 + p) * 18) * S + i;
+    Cplx<T>* b = reinterpret_cast<Cplx<T>*>(f) + (size_t)((mu * 2 + p) * 9) * S + i;
 #pragma unroll
-    for (int k = 0; k < 18; ++k) b[(size_t)k * S] = m.v[k];
+    for (int k = 0; k < 9; ++k) b[(size_t)k * S] = Cplx<T>{m.v[2 * k], m.v[2 * k + 1]};
 }
 
 // Block index -> work tile such that each XCD owns one contiguous range of

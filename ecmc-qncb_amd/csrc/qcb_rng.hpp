@@ -4,10 +4,13 @@
 // Counter scheme (the oracle restates the same one):
 //   key  = { seed & 0xffffffff, seed >> 32 }
 //   heatbath:  ctr = { cb_site, sweep_idx, (hs<<24)|(hit<<8)|(subgroup<<4), call }
-//              hs = 2*mu + parity; call 2a and 2a+1 feed Kennedy-Pendleton attempt a
-//              with (delta, r1) and (r2, r3); call 0x80000000 feeds the sphere draw.
-//   hot start: ctr = { lex_site, mu, 1, call 0..8 } (nine Box-Muller pairs)
-//   uniform  = u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) + 2^-53, strictly in (0,1).
+//              hs = 2*mu + parity; call a feeds Kennedy-Pendleton attempt a with
+//              (delta, r1, r2, r3) = u32(w0..w3); call 0x80000000 feeds the sphere
+//              draw with u32(w0), u32(w1).  u32(w) = (w + 1/2) 2^-32 is exact in
+//              double; the float path rounds that double, so FP32 and FP64 lattices
+//              consume identical uniforms.
+//   hot start: ctr = { lex_site, mu, 1, call 0..8 } (nine Box-Muller pairs) with
+//              u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) + 2^-53, strictly in (0,1).
 // Every number depends only on (seed, counter), never on launch shape or on
 // other lattices in a batch.
 #pragma once
@@ -39,6 +42,8 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
     uint64_t x = ((uint64_t)hi << 32) | lo;
     return (double)(x >> 12) * 0x1.0p-52 + 0x1.0p-53;
 }
+
+__device__ __forceinline__ double u32d(uint32_t w) { return ((double)w + 0.5) * 0x1.0p-32; }
 
 __host__ __device__ __forceinline__ uint32_t hb_ctr_z(int hs, int hit, int g) {
     return ((uint32_t)hs << 24) | ((uint32_t)hit << 8) | ((uint32_t)g << 4);

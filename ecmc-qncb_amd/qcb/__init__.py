@@ -45,6 +45,7 @@ SIGNATURES = [
     ("qcb_lattice_create", C.c_int, [_VP, C.POINTER(C.c_int), C.c_int, C.POINTER(_VP)]),
     ("qcb_lattice_destroy", C.c_int, [_VP]),
     ("qcb_lattice_bytes", C.c_int, [_VP, C.POINTER(C.c_uint64)]),
+    ("qcb_lattice_set_tiling", C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
     ("qcb_lattice_set_links", C.c_int, [_VP, _D]),
     ("qcb_lattice_get_links", C.c_int, [_VP, _D]),
     ("qcb_cold", C.c_int, [_VP]),
@@ -62,6 +63,20 @@ SIGNATURES = [
     ("qcb_flow_qe", C.c_int, [_VP, _D, _D]),
     ("qcb_flow_get_links", C.c_int, [_VP, _D]),
     ("qcb_topo_charge_flowed", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
+    ("qcb_batch_create", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.c_int, _D, C.POINTER(C.c_uint64),
+                                    C.POINTER(_VP)]),
+    ("qcb_batch_destroy", C.c_int, [_VP]),
+    ("qcb_batch_lattice", C.c_int, [_VP, C.c_int, C.POINTER(_VP)]),
+    ("qcb_batch_hot", C.c_int, [_VP]),
+    ("qcb_batch_hb_sweeps", C.c_int, [_VP, C.c_int, C.c_int, C.c_uint32]),
+    ("qcb_batch_reunitarize", C.c_int, [_VP]),
+    ("qcb_batch_plaquette", C.c_int, [_VP, _D]),
+    ("qcb_batch_flow", C.c_int, [_VP, C.c_double, C.c_int, _D]),
+    ("qcb_batch_set_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.c_uint32]),
+    ("qcb_batch_get_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
+    ("qcb_batch_autotune", C.c_int, [_VP, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    ("qcb_batch_graph_capture", C.c_int, [_VP, C.c_int, C.c_int, C.c_uint32]),
+    ("qcb_batch_graph_launch", C.c_int, [_VP, C.c_int]),
     ("qcb_ildg_write", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
     ("qcb_ildg_read", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
     ("qcb_ildg_save", C.c_int, [_VP, C.c_char_p]),
@@ -177,6 +192,9 @@ class Lattice:
         self._c(lib().qcb_lattice_get_links(self.h, _dp(out)))
         return out.view(np.complex128).reshape(self.V, 4, 3, 3)
 
+    def set_tiling(self, ty: int, tz: int, tt: int):
+        self._c(lib().qcb_lattice_set_tiling(self.h, ty, tz, tt))
+
     def nbytes(self) -> int:
         b = C.c_uint64()
         self._c(lib().qcb_lattice_bytes(self.h, C.byref(b)))
@@ -253,3 +271,92 @@ class Lattice:
                                              _dp(fin) if final else None))
         t = tqe[: 3 * n_steps_gf].reshape(n_steps_gf, 3)
         return (t, fin) if final else t
+
+
+class _Borrowed(Lattice):
+    """A lattice owned by a Batch (never destroyed through this handle)."""
+
+    def __init__(self, batch, i, handle, dims, prec):
+        self.ctx = batch.ctx
+        self.dims = tuple(dims)
+        self.prec = prec
+        self.V = int(np.prod(self.dims))
+        self.h = handle
+
+    def close(self):
+        self.h = _VP()
+
+
+class Batch:
+    """Independent lattices advanced together by single launches."""
+
+    def __init__(self, ctx: Context, dims_list, prec: int, betas, seeds):
+        self.ctx = ctx
+        self.prec = prec
+        self.dims = [tuple(int(v) for v in d) for d in dims_list]
+        n = len(self.dims)
+        self.n = n
+        flat = (C.c_int * (4 * n))(*[v for d in self.dims for v in d])
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        sd = (C.c_uint64 * n)(*[int(x) for x in seeds])
+        self.h = _VP()
+        ctx.check(lib().qcb_batch_create(ctx.h, n, flat, prec, _dp(b), sd, C.byref(self.h)))
+        ctx._children.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().qcb_batch_destroy(self.h)
+        self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        return self.ctx.check(rc)
+
+    def lattice(self, i) -> Lattice:
+        h = _VP()
+        self._c(lib().qcb_batch_lattice(self.h, i, C.byref(h)))
+        return _Borrowed(self, i, h, self.dims[i], self.prec)
+
+    def hot(self):
+        self._c(lib().qcb_batch_hot(self.h))
+
+    def hb_sweeps(self, n_hits, n_sweeps, sweep_idx=0):
+        self._c(lib().qcb_batch_hb_sweeps(self.h, n_hits, n_sweeps, sweep_idx))
+
+    def reunitarize(self):
+        self._c(lib().qcb_batch_reunitarize(self.h))
+
+    def plaquette(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        self._c(lib().qcb_batch_plaquette(self.h, _dp(out)))
+        return out
+
+    def flow(self, eps, n_steps) -> np.ndarray:
+        out = np.zeros(self.n * (n_steps + 1) * 3)
+        self._c(lib().qcb_batch_flow(self.h, eps, n_steps, _dp(out)))
+        return out.reshape(self.n, n_steps + 1, 3)
+
+    def set_schedule(self, order, grid=0):
+        self._c(lib().qcb_batch_set_schedule(self.h, (C.c_int * 8)(*order), grid))
+
+    def get_schedule(self):
+        o = (C.c_int * 8)()
+        g = C.c_uint32()
+        self._c(lib().qcb_batch_get_schedule(self.h, o, C.byref(g)))
+        return list(o), g.value
+
+    def autotune(self, n_hits=2, reps=2) -> str:
+        buf = C.create_string_buffer(4096)
+        self._c(lib().qcb_batch_autotune(self.h, n_hits, reps, buf, len(buf)))
+        return buf.value.decode()
+
+    def graph_capture(self, n_hits, sweeps_per_graph, first_sweep=0):
+        self._c(lib().qcb_batch_graph_capture(self.h, n_hits, sweeps_per_graph, first_sweep))
+
+    def graph_launch(self, n_replays=1):
+        self._c(lib().qcb_batch_graph_launch(self.h, n_replays))

@@ -64,6 +64,11 @@ int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb_lattice** 
 int qcb_lattice_destroy(qcb_lattice* lat);
 /* Device bytes held by the handle (links plus flow buffers once allocated). */
 int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes);
+/* Work-order tiling of the neighbour-reading kernels: tiles of Lx x ty x tz x tt
+ * sites (each must divide its extent).  Changes speed only: every kernel
+ * result is independent of it except the last bits of the FP64 global sums,
+ * whose block partition follows the tiles.  Default: largest divisor <= 4. */
+int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt);
 /* Upload / download of the whole field (src/io/ildg.cpp:87-106,213-232 minus
  * the byte swap, which stays with the caller's file codec). */
 int qcb_lattice_set_links(qcb_lattice* lat, const double* aos);
@@ -129,6 +134,36 @@ int qcb_flow_get_links(qcb_lattice* lat, double* aos);
  * after the last step (the reference never measures it). */
 int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk_steps,
                            double* tQE, double* final_tQE);
+
+/* ---- batches of independent lattices ------------------------------------ */
+/* n lattices of uneven extents (dims[4*i..4*i+3]), each with its own beta and
+ * Philox seed, advanced together: one launch per (mu, parity) half-sweep,
+ * per RK3 stage and per reduction covers every lattice (blocks never span
+ * two lattices, so each lattice's results are bitwise those of running it
+ * alone).  The reference runs one lattice per process; this replaces many
+ * concurrent runs of src/bin/gauge_hb_ncb.cpp. */
+int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, const double* betas,
+                     const uint64_t* seeds, qcb_batch** out);
+int qcb_batch_destroy(qcb_batch* b);
+/* Borrowed handle of lattice i (set/get links, observables); owned by the batch. */
+int qcb_batch_lattice(qcb_batch* b, int i, qcb_lattice** out);
+int qcb_batch_hot(qcb_batch* b);
+int qcb_batch_hb_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx);
+int qcb_batch_reunitarize(qcb_batch* b);
+int qcb_batch_plaquette(qcb_batch* b, double* out /* n */);
+/* Flow all lattices from their links, measuring (t, Q, E_mean) at t = 0, eps,
+ * ..., n_steps*eps: tQE[((i*(n_steps+1)) + s)*3 + {0,1,2}]. */
+int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE);
+/* Launch schedule of the heatbath: the order of the 8 half-sweeps (hs = 2*mu +
+ * parity; any permutation is a valid Gibbs sweep) and a persistent grid size
+ * (0 = one block per tile, else a multiple of 8 blocks striding over tiles). */
+int qcb_batch_set_schedule(qcb_batch* b, const int order[8], uint32_t hb_grid);
+int qcb_batch_get_schedule(qcb_batch* b, int order[8], uint32_t* hb_grid);
+/* Times candidate schedules on scratch copies and keeps the fastest. */
+int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len);
+/* hipGraph of sweeps_per_graph sweeps; replays continue the Philox sweep index. */
+int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_graph, uint32_t first_sweep);
+int qcb_batch_graph_launch(qcb_batch* b, int n_replays);
 
 /* ---- ILDG / LIME configuration files ----------------------------------- */
 /* The file layout of save_ildg_clime / read_ildg_clime (src/io/ildg.cpp:13-237):

@@ -56,6 +56,7 @@ uint64_t qo_mt64_uniform_int(qo_mt64* g, uint64_t a, uint64_t b); /* std::unifor
 void qo_mt64_normal_pair(qo_mt64* g, double* first, double* second); /* std::normal_distribution: two draws */
 void qo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 double qo_u53(uint32_t hi, uint32_t lo);
+double qo_u32(uint32_t w);
 
 /* ---------------- lattice ---------------- */
 typedef struct {

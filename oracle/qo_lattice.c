@@ -553,18 +553,14 @@ static int kp_phx_cb(void* vctx, int g, double beta, double k, double out[4]) {
     uint32_t c2 = (c->hs << 24) | ((uint32_t)c->h << 8) | ((uint32_t)g << 4);
     uint32_t a;
     for (a = 0; a < 1024; ++a) {   /* same attempt cap as the device kernel */
-        uint32_t ctr0[4] = {c->cb, c->sweep, c2, 2 * a}, ctr1[4] = {c->cb, c->sweep, c2, 2 * a + 1};
-        uint32_t o0[4], o1[4];
-        qo_philox4x32_10(ctr0, c->key, o0);
-        qo_philox4x32_10(ctr1, c->key, o1);
-        if (qo_kp_try(alpha, qo_u53(o0[0], o0[1]), qo_u53(o0[2], o0[3]), qo_u53(o1[0], o1[1]),
-                      qo_u53(o1[2], o1[3]), &a0))
-            break;
+        uint32_t ctr[4] = {c->cb, c->sweep, c2, a}, o[4];
+        qo_philox4x32_10(ctr, c->key, o);
+        if (qo_kp_try(alpha, qo_u32(o[0]), qo_u32(o[1]), qo_u32(o[2]), qo_u32(o[3]), &a0)) break;
     }
     if (a == 1024) return -1;
     uint32_t cs[4] = {c->cb, c->sweep, c2, 0x80000000u}, os[4];
     qo_philox4x32_10(cs, c->key, os);
-    qo_kp_sphere(a0, qo_u53(os[0], os[1]), qo_u53(os[2], os[3]), out);
+    qo_kp_sphere(a0, qo_u32(os[0]), qo_u32(os[1]), out);
     return 0;
 }
 

@@ -13,10 +13,14 @@
  * Philox counter scheme (identical in ecmc-qncb_amd/csrc/qcb_rng.hpp):
  *   key  = { seed & 0xffffffff, seed >> 32 }
  *   heatbath: ctr = { cb_site, sweep_idx, (hs<<24)|(hit<<8)|(subgroup<<4)|0, call }
- *             call 2a, 2a+1 -> attempt a (delta,r1),(r2,r3); call 0x80000000 -> sphere
+ *             call a -> attempt a: (delta, r1, r2, r3) = u32(w0..w3);
+ *             call 0x80000000 -> sphere: (phi, cos theta) = u32(w0), u32(w1)
+ *             u32(w) = (w + 1/2) 2^-32, exact in double and strictly inside (0,1);
+ *             the float path rounds the same double, so both precisions see
+ *             the same uniforms.
  *   hot start: ctr = { lex_site, mu, 1, call 0..8 }  (Box-Muller pairs)
- *   uniform  = u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) plus half an ulp,
- *              so every value lies strictly inside (0,1).
+ *              with u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) plus half
+ *              an ulp, strictly inside (0,1).
  */
 #include "qo.h"
 
@@ -114,6 +118,8 @@ double qo_u53(uint32_t hi, uint32_t lo) {
     uint64_t x = ((uint64_t)hi << 32) | lo;
     return (double)(x >> 12) * 0x1.0p-52 + 0x1.0p-53;
 }
+
+double qo_u32(uint32_t w) { return ((double)w + 0.5) * 0x1.0p-32; }
 
 double qo_now(void) {
     struct timespec ts;
