@@ -74,6 +74,44 @@ def time_wf(qcb, ctx, L, prec, steps, warmup, eps=0.02, seed=11):
                 avg_stage_launch_us=launch_s * 1e6, achieved_gbs=achieved)
 
 
+def serve(qcb, ctx, n_req=60, rate=20.0, n_therm=50, seed=2026):
+    """Flow-measurement service under Poisson arrivals: L in {12,16,24},
+    T = 2L, beta in {5.9,6.0,6.1}, t_max ~ U[0.5, 2.0], eps = 0.02, FP32 links
+    with FP64 sums.  Latency = result ready - scheduled arrival."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    Ls = rng.choice([12, 16, 24], size=n_req)
+    betas = rng.choice([5.9, 6.0, 6.1], size=n_req)
+    tmax = np.round(rng.uniform(0.5, 2.0, size=n_req) / 0.02) * 0.02
+    svc = qcb.Service(ctx, qcb.F32, 64 << 30, 2, n_therm, 4, 0.02)
+    t0 = svc.clock()
+    nxt, done, lat = 0, {}, {}
+    while len(done) < n_req:
+        now = svc.clock() - t0
+        while nxt < n_req and arrivals[nxt] <= now:
+            L = int(Ls[nxt])
+            svc.submit(nxt, (L, L, L, 2 * L), float(betas[nxt]), 1000 + nxt, float(tmax[nxt]), t0 + arrivals[nxt])
+            nxt += 1
+        st = svc.status()
+        if st["queued"] + st["resident"] == 0:
+            time.sleep(max(0.0, min(0.01, arrivals[nxt] - now)) if nxt < n_req else 0.0)
+            continue
+        svc.step()
+        for rid, tqe, (ta, tadm, td) in svc.poll():
+            done[rid] = tqe
+            lat[rid] = td - ta
+    t_end = svc.clock() - t0
+    it = svc.status()["iterations"]
+    svc.close()
+    l = np.array([lat[i] for i in range(n_req)])
+    sites = sum(int(L) ** 3 * 2 * int(L) for L in Ls)
+    return dict(requests=n_req, rate_per_s=rate, p50_ms=float(np.percentile(l, 50) * 1e3),
+                p99_ms=float(np.percentile(l, 99) * 1e3), mean_ms=float(l.mean() * 1e3),
+                requests_per_s=n_req / t_end, wall_s=t_end, iterations=int(it), n_therm=n_therm,
+                mean_sites=sites / n_req)
+
+
 def cpu_baseline(L=32):
     """One reference-order FP64 sweep (N_hits = 2, beta = 6.0) of a hot L^4
     lattice on one core; counts only the non-frozen links it updates."""
@@ -99,6 +137,8 @@ def main():
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -121,6 +161,12 @@ def main():
         wf64 = time_wf(qcb, ctx, L, qcb.F64, max(2, args.flow_steps // 2), 1)
         log(f"[bench] WF FP64 {wf64['site_steps_per_s']:.3e} site-steps/s, {wf64['achieved_gbs']:.0f} GB/s")
         comp.update(hb_fp64=hb64, wf_fp64=wf64)
+        comp["fp32_over_fp64_hb"] = hb32["link_updates_per_s"] / hb64["link_updates_per_s"]
+    if not args.no_serve:
+        log(f"[bench] flow service: {args.serve_requests} Poisson requests at 20/s")
+        sv = serve(qcb, ctx, args.serve_requests)
+        log(f"[bench] service p50 {sv['p50_ms']:.1f} ms p99 {sv['p99_ms']:.1f} ms, {sv['requests_per_s']:.2f} req/s")
+        comp["flow_service"] = sv
     ctx.close()
 
     cpu = None

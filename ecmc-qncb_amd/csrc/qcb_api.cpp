@@ -110,7 +110,7 @@ extern "C" int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb
     l->Vh = (uint32_t)(V / 2);
     l->S = (l->Vh + 63u) & ~63u;
     l->esize = prec == QCB_F64 ? 8 : 4;
-    for (int d = 1; d < 4; ++d) l->tile[d] = default_tile(dims[d]);
+    default_tiling(dims, l->tile);
     int rc = lattice_alloc(l);
     if (rc != QCB_OK) {
         lattice_free(l);

@@ -38,6 +38,7 @@ struct qcb_lattice {
     uint32_t gstep = 0, flow_start = 0;
     bool flow_active = false;   // flow buffers hold a GradientFlow state (qcb_flow_copy)
     int tile[4] = {0, 1, 1, 1};  // work-order tile extents (tile[0] unused: whole x rows)
+    uint32_t sweep_off = 0;      // per-lattice offset of the Philox sweep index
 };
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
@@ -61,6 +62,14 @@ inline int default_tile(int L) {
     return 1;
 }
 
+// Default tiling: whole y rows (y up to 32), 4 x 4 in z and t when they divide.
+inline void default_tiling(const int L[4], int tile[4]) {
+    tile[0] = 0;
+    tile[1] = L[1] <= 32 ? L[1] : default_tile(L[1]);
+    tile[2] = default_tile(L[2]);
+    tile[3] = default_tile(L[3]);
+}
+
 // Blocks one lattice contributes to a launch of the given kernel family.
 inline uint32_t fam_blocks(uint32_t Vh, int fam) {
     const uint32_t per = fam == kFamHalf ? kHalfSites : fam == kFamSite ? kSiteSites : kLinkSites;
@@ -80,6 +89,7 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
     d.key0 = (uint32_t)l->seed;
     d.key1 = (uint32_t)(l->seed >> 32);
     d.flow_start = l->flow_start;
+    d.sweep_off = l->sweep_off;
     d.TL[0] = l->L[0];
     d.NT[0] = 1;
     for (int k = 1; k < 4; ++k) {

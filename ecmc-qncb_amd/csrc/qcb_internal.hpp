@@ -27,6 +27,7 @@ template <class T> void launch_hot(const Launch& g);   // key from descriptor
 template <class T> void launch_aos_to_soa(const Launch& g, const double* aos, T* dst_field);
 template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
 template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
+template <class T> void launch_flow_init(const Launch& g);   // F[0] <- U for every lattice
 template <class T> void launch_plaquette(const Launch& g, double* partial);
 template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
 template <class T> void launch_staples(const Launch& g, T* out_field);

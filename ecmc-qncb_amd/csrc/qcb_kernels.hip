@@ -193,6 +193,18 @@ __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restri
     st_link(dst, L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(src, L.S, (int)(hs >> 1), (int)(hs & 1), i));
 }
 
+// Flow start for every lattice of a batch: F[0] <- U (GradientFlow::copy).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_flow_init(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+                                                      int n_lat, uint32_t nblocks) {
+    BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
+    const LatDev<T>& L = *bc.L;
+    const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t hs = threadIdx.x / kLinkSites;
+    if (i >= L.Vh) return;
+    st_link(L.F[0], L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i));
+}
+
 // U'(x) = U(x - s): each thread writes one destination link.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_translate(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
@@ -425,14 +437,30 @@ __global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict_
 
 // ------------------------------------------------------------- heatbath ----
 
-__device__ __forceinline__ void sin_cos(float x, float* s, float* c) { sincosf(x, s, c); }
-__device__ __forceinline__ void sin_cos(double x, double* s, double* c) { sincos(x, s, c); }
+// Transcendentals of the Kennedy-Pendleton step.  The double path calls the
+// accurate libm routines (it is held to 1e-12 against the oracle); the float
+// path uses the gfx950 hardware approximations, whose error (~1 ulp of float)
+// is far below the statistical resolution of a single draw.  cos2pi(u) and
+// sin2pi(u) take the angle in turns, which is what v_cos_f32 / v_sin_f32 take.
+struct KpMath {
+    __device__ static __forceinline__ double ln(double x) { return log(x); }
+    __device__ static __forceinline__ double cos2pi(double u) { return cos(2.0 * M_PI * u); }
+    __device__ static __forceinline__ void sincos2pi(double u, double* s, double* c) { sincos(2.0 * M_PI * u, s, c); }
+    __device__ static __forceinline__ double sq(double x) { return sqrt(x); }
+    __device__ static __forceinline__ float ln(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+    __device__ static __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
+    __device__ static __forceinline__ void sincos2pi(float u, float* s, float* c) {
+        *s = __builtin_amdgcn_sinf(u);
+        *c = __builtin_amdgcn_cosf(u);
+    }
+    __device__ static __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
 
 // One Kennedy-Pendleton attempt (reference src/heatbath/heatbath.cpp:14-26).
 template <class T>
 __device__ __forceinline__ bool kp_try(T alpha, T delta, T r1, T r2, T r3, T& a0) {
-    T cs = cos(T(2.0 * M_PI) * r2);
-    T lam2 = T(-0.5) / alpha * (log(r1) + cs * cs * log(r3));
+    T cs = KpMath::cos2pi(r2);
+    T lam2 = T(-0.5) / alpha * (KpMath::ln(r1) + cs * cs * KpMath::ln(r3));
     if (delta * delta <= T(1) - lam2) {
         a0 = T(1) - T(2) * lam2;
         return true;
@@ -443,12 +471,11 @@ __device__ __forceinline__ bool kp_try(T alpha, T delta, T r1, T r2, T r3, T& a0
 // Sphere draw of the remaining three components (reference src/heatbath/heatbath.cpp:29-35).
 template <class T>
 __device__ __forceinline__ void kp_sphere(T a0, T uphi, T ucos, T x[4]) {
-    T rs = sqrt(T(1) - a0 * a0);
-    T phi = T(2.0 * M_PI) * uphi;
+    T rs = KpMath::sq(T(1) - a0 * a0);
     T ct = T(2) * ucos - T(1);
-    T st = sqrt(T(1) - ct * ct);
+    T st = KpMath::sq(T(1) - ct * ct);
     T sp, cp;
-    sin_cos(phi, &sp, &cp);
+    KpMath::sincos2pi(uphi, &sp, &cp);
     x[0] = a0;
     x[1] = rs * st * cp;
     x[2] = rs * st * sp;
@@ -489,10 +516,11 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
                 w[(rr * 2 + cc) * 2 + 1] = im;
             }
         // su2_to_quaternion (reference src/su3/utils.cpp:47-50)
-        T q0 = (w[0] + w[6]) / T(2), q1 = (w[3] + w[5]) / T(2);
-        T q2 = (w[2] - w[4]) / T(2), q3 = (w[1] - w[7]) / T(2);
-        T k = sqrt(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
-        T v0 = q0 / k, v1 = q1 / k, v2 = q2 / k, v3 = q3 / k;
+        T q0 = (w[0] + w[6]) * T(0.5), q1 = (w[3] + w[5]) * T(0.5);
+        T q2 = (w[2] - w[4]) * T(0.5), q3 = (w[1] - w[7]) * T(0.5);
+        T k = KpMath::sq(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
+        T ik = T(1) / k;
+        T v0 = q0 * ik, v1 = q1 * ik, v2 = q2 * ik, v3 = q3 * ik;
         T alpha = beta * k * T(2) / T(3);
         T x[4];
         bool ok = false;
@@ -553,9 +581,21 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
 // staple once and applies n_hits CM hits in registers.
 // Occupancy floor of the heavy kernels: with no floor the scheduler hoists
 // every neighbour load and the double kernels fall to one wave per SIMD.
+#ifndef QCB_HB_WAVES_F32
+#define QCB_HB_WAVES_F32 3
+#endif
+#ifndef QCB_FLOW_WAVES_F32
+#define QCB_FLOW_WAVES_F32 4
+#endif
+#ifndef QCB_HB_WAVES_F64
+#define QCB_HB_WAVES_F64 2
+#endif
+#ifndef QCB_FLOW_WAVES_F64
+#define QCB_FLOW_WAVES_F64 2
+#endif
 template <class T> struct MinWaves;
-template <> struct MinWaves<float> { static constexpr int hb = 3, flow = 4; };
-template <> struct MinWaves<double> { static constexpr int hb = 2, flow = 2; };
+template <> struct MinWaves<float> { static constexpr int hb = QCB_HB_WAVES_F32, flow = QCB_FLOW_WAVES_F32; };
+template <> struct MinWaves<double> { static constexpr int hb = QCB_HB_WAVES_F64, flow = QCB_FLOW_WAVES_F64; };
 
 template <class T, int MU, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::hb))) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
@@ -579,9 +619,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         M3<T> A = staple<T, MU>(L, L.U, c, p, i);
         M3<T> U = ld_link(L.U, L.S, MU, p, i);
         const T beta = (T)L.beta;
+        const uint32_t sw = sweep + L.sweep_off;
         int ex = 0;
         for (int h = 0; h < n_hits; ++h)
-            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sweep, L.key0, L.key1, inj, n_hits, K, L.Vh);
+            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh);
         st_link(L.U, L.S, MU, p, i, U);
         if (ex) atomicAdd(exhausted, (unsigned long long)ex);
     }
@@ -726,6 +767,10 @@ void launch_copy_field(const Launch& g, const T* src, T* dst) {
     hipLaunchKernelGGL(k_copy_field<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, dst);
 }
 template <class T>
+void launch_flow_init(const Launch& g) {
+    hipLaunchKernelGGL(k_flow_init<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+}
+template <class T>
 void launch_plaquette(const Launch& g, double* partial) {
     hipLaunchKernelGGL(k_plaquette<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), partial);
 }
@@ -806,6 +851,7 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_aos_to_soa<T>(const Launch&, const double*, T*);                                     \
     template void launch_soa_to_aos<T>(const Launch&, const T*, double*);                                     \
     template void launch_copy_field<T>(const Launch&, const T*, T*);                                          \
+    template void launch_flow_init<T>(const Launch&);                                                         \
     template void launch_plaquette<T>(const Launch&, double*);                                                \
     template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
     template void launch_staples<T>(const Launch&, T*);                                                       \

@@ -1,16 +1,18 @@
 // qcb_layout.hpp -- device link storage and periodic even/odd indexing.
 //
-// HBM layout (one lattice): structure-of-arrays over complex entries,
-// even/odd split,
-//   field[((mu*2 + parity)*9 + k)*S + cb]   (one complex = 2 reals)
-// k = 3*i + j, cb = lex/2 with lex = x + Lx*(y + Ly*(z + Lz*t)),
-// parity = (x+y+z+t) & 1 and S = Vh rounded up to 64.  A 64-lane wavefront
-// reading entry k of 64 consecutive cb sites issues one fully coalesced
-// 512-byte (float) / 1 KiB (double) load of 8/16-byte elements.  The
-// reference keeps an AoS field with a one-site halo and a neighbour table
-// (src/gauge/GaugeField.h:27, src/geometry/GeometryCB.cpp:10-31); here
-// neighbours are closed-form periodic index arithmetic, so the halo exchange
-// of src/mpi/HalosExchange.cpp has no counterpart.
+// HBM layout (one lattice): blocks of 64 sites, complex entries innermost
+// across the block (AoSoA), even/odd split,
+//   field[((hs*NB + cb/64)*9 + k)*64 + cb%64]   (one complex = 2 reals)
+// hs = 2*mu + parity, k = 3*i + j, cb = lex/2 with lex = x + Lx*(y + Ly*(z +
+// Lz*t)), parity = (x+y+z+t) & 1, NB = S/64 and S = Vh rounded up to 64.  A
+// 64-lane wavefront reading entry k of 64 consecutive cb sites issues one
+// fully coalesced 512-byte (float) / 1 KiB (double) load, and the 9 entries
+// of one link sit at fixed 512 B / 1 KiB offsets from each other, so a link
+// costs one address computation.  The reference keeps an AoS field with a
+// one-site halo and a neighbour table (src/gauge/GaugeField.h:27,
+// src/geometry/GeometryCB.cpp:10-31); here neighbours are closed-form
+// periodic index arithmetic, so the halo exchange of src/mpi/HalosExchange.cpp
+// has no counterpart.
 //
 // Work order: kernels that read neighbours visit sites in a 4D-tiled order
 // (tiles of Lx x T1 x T2 x T3 sites, tiles y-fastest), so that the +-t and +-z
@@ -41,6 +43,7 @@ struct LatDev {
     double beta;
     uint32_t key0, key1;  // Philox key (seed)
     uint32_t flow_start;  // global flow-step index at which this lattice's flow began
+    uint32_t sweep_off;   // added to the launch's Philox sweep index (per-lattice sweep count)
     int TL[4];            // tile extents in sites (TL[0] = Lx: whole x rows)
     int NT[4];            // tiles per direction (NT[0] = 1)
     uint32_t tvol;        // cb sites per tile = (Lx/2) * T1 * T2 * T3
@@ -122,13 +125,18 @@ struct alignas(2 * sizeof(T)) Cplx {
     T re, im;
 };
 
+// Complex-entry index of entry 0 of link (hs, i); entry k is 64*k further.
+__device__ __forceinline__ size_t link_base(uint32_t S, int hs, uint32_t i) {
+    return ((size_t)hs * (S >> 6) + (i >> 6)) * 576u + (i & 63u);
+}
+
 template <class T>
 __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, uint32_t i) {
-    const Cplx<T>* b = reinterpret_cast<const Cplx<T>*>(f) + (size_t)((mu * 2 + p) * 9) * S + i;
+    const Cplx<T>* b = reinterpret_cast<const Cplx<T>*>(f) + link_base(S, mu * 2 + p, i);
     M3<T> m;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        Cplx<T> c = b[(size_t)k * S];
+        Cplx<T> c = b[64 * k];
         m.v[2 * k] = c.re;
         m.v[2 * k + 1] = c.im;
     }
@@ -137,9 +145,9 @@ __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, 
 
 template <class T>
 __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_t i, const M3<T>& m) {
-    Cplx<T>* b = reinterpret_cast<Cplx<T>*>(f) + (size_t)((mu * 2 + p) * 9) * S + i;
+    Cplx<T>* b = reinterpret_cast<Cplx<T>*>(f) + link_base(S, mu * 2 + p, i);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) b[(size_t)k * S] = Cplx<T>{m.v[2 * k], m.v[2 * k + 1]};
+    for (int k = 0; k < 9; ++k) b[64 * k] = Cplx<T>{m.v[2 * k], m.v[2 * k + 1]};
 }
 
 // Block index -> work tile such that each XCD owns one contiguous range of

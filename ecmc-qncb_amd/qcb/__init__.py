@@ -16,7 +16,8 @@ import weakref
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libqcb.so")
+# QCB_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("QCB_LIBRARY") or os.path.join(PKG_DIR, "libqcb.so")
 
 F64, F32 = 0, 1
 OK, EINVAL, EDEVICE, ENOMEM, EBUDGET, ESTATE = 0, -1, -2, -3, -4, -5
@@ -77,6 +78,16 @@ SIGNATURES = [
     ("qcb_batch_autotune", C.c_int, [_VP, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     ("qcb_batch_graph_capture", C.c_int, [_VP, C.c_int, C.c_int, C.c_uint32]),
     ("qcb_batch_graph_launch", C.c_int, [_VP, C.c_int]),
+    ("qcb_service_create", C.c_int, [_VP, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.POINTER(_VP)]),
+    ("qcb_service_destroy", C.c_int, [_VP]),
+    ("qcb_service_clock", C.c_double, [_VP]),
+    ("qcb_service_submit", C.c_int, [_VP, C.c_uint64, C.POINTER(C.c_int), C.c_double, C.c_uint64, C.c_double,
+                                      C.c_double]),
+    ("qcb_service_step", C.c_int, [_VP, C.POINTER(C.c_int)]),
+    ("qcb_service_poll", C.c_int, [_VP, C.POINTER(C.c_uint64), _D, C.c_int, C.POINTER(C.c_int), _D]),
+    ("qcb_service_status", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("qcb_ildg_write", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
     ("qcb_ildg_read", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
     ("qcb_ildg_save", C.c_int, [_VP, C.c_char_p]),
@@ -360,3 +371,59 @@ class Batch:
 
     def graph_launch(self, n_replays=1):
         self._c(lib().qcb_batch_graph_launch(self.h, n_replays))
+
+
+class Service:
+    """Continuous-batching flow-measurement service (see include/qcb.h)."""
+
+    def __init__(self, ctx: Context, prec=F32, arena_bytes=16 << 30, n_hits=2, n_therm=50, reunit_every=4,
+                 eps=0.02):
+        self.ctx = ctx
+        self.eps = eps
+        self.h = _VP()
+        ctx.check(lib().qcb_service_create(ctx.h, prec, int(arena_bytes), n_hits, n_therm, reunit_every, eps,
+                                           C.byref(self.h)))
+        ctx._children.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().qcb_service_destroy(self.h)
+        self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clock(self) -> float:
+        return lib().qcb_service_clock(self.h)
+
+    def submit(self, rid, dims, beta, seed, t_max, t_arrival=None):
+        if t_arrival is None:
+            t_arrival = self.clock()
+        self.ctx.check(lib().qcb_service_submit(self.h, rid, (C.c_int * 4)(*dims), beta, seed, t_max, t_arrival))
+
+    def step(self) -> int:
+        p = C.c_int()
+        self.ctx.check(lib().qcb_service_step(self.h, C.byref(p)))
+        return p.value
+
+    def poll(self, cap=4096):
+        """Finished requests as a list of (id, tQE[n,3], (arrival, admit, done))."""
+        out = []
+        while True:
+            rid, n = C.c_uint64(), C.c_int()
+            buf = np.zeros(cap * 3)
+            times = np.zeros(3)
+            rc = lib().qcb_service_poll(self.h, C.byref(rid), _dp(buf), cap, C.byref(n), _dp(times))
+            if rc == ESTATE:
+                return out
+            self.ctx.check(rc)
+            out.append((rid.value, buf[: 3 * n.value].reshape(n.value, 3), tuple(times)))
+
+    def status(self):
+        q, r, f = C.c_int(), C.c_int(), C.c_int()
+        fb, it = C.c_uint64(), C.c_uint64()
+        self.ctx.check(lib().qcb_service_status(self.h, C.byref(q), C.byref(r), C.byref(f), C.byref(fb), C.byref(it)))
+        return dict(queued=q.value, resident=r.value, finished=f.value, free_bytes=fb.value, iterations=it.value)

@@ -67,7 +67,8 @@ int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes);
 /* Work-order tiling of the neighbour-reading kernels: tiles of Lx x ty x tz x tt
  * sites (each must divide its extent).  Changes speed only: every kernel
  * result is independent of it except the last bits of the FP64 global sums,
- * whose block partition follows the tiles.  Default: largest divisor <= 4. */
+ * whose block partition follows the tiles.  Default: whole y rows (Ly <= 32),
+ * 4 x 4 in z and t (largest divisors <= 4). */
 int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt);
 /* Upload / download of the whole field (src/io/ildg.cpp:87-106,213-232 minus
  * the byte swap, which stays with the caller's file codec). */
@@ -164,6 +165,33 @@ int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t 
 /* hipGraph of sweeps_per_graph sweeps; replays continue the Philox sweep index. */
 int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_graph, uint32_t first_sweep);
 int qcb_batch_graph_launch(qcb_batch* b, int n_replays);
+
+/* ---- flow-measurement service (continuous batching) ---------------------- */
+/* A request = a lattice (dims, beta, seed) generated on the device (Philox
+ * hot start + n_therm heatbath sweeps, N_hits = n_hits, reunitarised every
+ * reunit_every sweeps) and flowed to t_max with (t, Q, E_mean) recorded at
+ * t = 0, eps, ..., t_max -- the schedule of topo_charge_flowed
+ * (src/observables/observables_mpi.cpp:153-178) plus the final state.
+ * Requests are admitted into a fixed arena of arena_bytes of HBM by footprint
+ * (four link fields each) and advanced together, one sweep or one flow step
+ * per service iteration, in batched launches on two streams.  Results do not
+ * depend on what else is resident (bitwise).  No counterpart in the
+ * reference, which runs one configuration per MPI job. */
+typedef struct qcb_service qcb_service;
+int qcb_service_create(qcb_ctx* ctx, int prec, uint64_t arena_bytes, int n_hits, int n_therm, int reunit_every,
+                       double eps, qcb_service** out);
+int qcb_service_destroy(qcb_service* s);
+/* Seconds since the service was created (the clock of arrival/admit/done times). */
+double qcb_service_clock(qcb_service* s);
+int qcb_service_submit(qcb_service* s, uint64_t id, const int dims[4], double beta, uint64_t seed, double t_max,
+                       double t_arrival);
+/* One iteration: admit, advance every resident request, retire finished ones. */
+int qcb_service_step(qcb_service* s, int* pending);
+/* Pops one finished request (QCB_ESTATE if none): tQE holds n_meas triples;
+ * times3 = (arrival, admit, done) on the service clock. */
+int qcb_service_poll(qcb_service* s, uint64_t* id, double* tQE, int cap, int* n_meas, double* times3);
+int qcb_service_status(qcb_service* s, int* queued, int* resident, int* finished, uint64_t* free_bytes,
+                       uint64_t* iterations);
 
 /* ---- ILDG / LIME configuration files ----------------------------------- */
 /* The file layout of save_ildg_clime / read_ildg_clime (src/io/ildg.cpp:13-237):

@@ -1,0 +1,65 @@
+"""Flow-measurement service: a request's (t, Q, E) series is bitwise the same
+alone and in a mix of other requests, and equals the single-lattice path
+(hot start, sweeps with reunitarisation every 4, topo_charge_flowed)."""
+import numpy as np
+import pytest
+
+import qcb
+
+pytestmark = pytest.mark.gpu
+
+N_THERM, EPS = 6, 0.02
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qcb.Context(0)
+    yield c
+    c.close()
+
+
+def run(svc, reqs):
+    """reqs: list of (id, dims, beta, seed, t_max, submit_after_iteration)"""
+    pending = sorted(reqs, key=lambda r: r[5])
+    out, it = {}, 0
+    while pending or svc.status()["resident"] or svc.status()["queued"]:
+        while pending and pending[0][5] <= it:
+            r = pending.pop(0)
+            svc.submit(r[0], r[1], r[2], r[3], r[4])
+        svc.step()
+        it += 1
+        for rid, tqe, times in svc.poll():
+            out[rid] = (tqe, times)
+        assert it < 10000
+    return out
+
+
+@pytest.mark.parametrize("prec", [qcb.F32, qcb.F64])
+def test_alone_equals_mixed(ctx, prec):
+    target = (7, (8, 8, 8, 16), 6.0, 99, 0.1, 0)
+    alone = run(qcb.Service(ctx, prec, 1 << 30, 2, N_THERM, 4, EPS), [target])[7][0]
+    mix = [(1, (12, 12, 12, 24), 5.9, 1, 0.14, 0), (2, (4, 4, 4, 8), 6.1, 2, 0.04, 1),
+           (7, (8, 8, 8, 16), 6.0, 99, 0.1, 3), (3, (8, 8, 8, 16), 6.0, 98, 0.2, 3),
+           (4, (16, 16, 16, 32), 6.0, 4, 0.06, 5)]
+    # a small arena forces queueing and first-fit admission
+    svc = qcb.Service(ctx, prec, 300 << 20, 2, N_THERM, 4, EPS)
+    got = run(svc, mix)
+    assert set(got) == {1, 2, 3, 4, 7}
+    assert np.array_equal(got[7][0], alone)
+    assert alone.shape == (6, 3)
+    for rid, (tqe, (ta, tadm, td)) in got.items():
+        assert ta <= tadm <= td
+
+
+def test_service_equals_single_lattice_path(ctx):
+    dims, beta, seed, t_max = (8, 8, 8, 16), 6.0, 5, 0.08
+    res = run(qcb.Service(ctx, qcb.F32, 1 << 30, 2, N_THERM, 4, EPS), [(1, dims, beta, seed, t_max, 0)])[1][0]
+    lat = qcb.Lattice(ctx, dims, qcb.F32)
+    lat.hot(seed)
+    for s in range(N_THERM):
+        lat.hb_sweeps(beta, 2, 1, seed=seed, sweep_idx=s)
+        if (s + 1) % 4 == 0:
+            lat.reunitarize()
+    t, fin = lat.topo_charge_flowed(EPS, 4, 1, final=True)
+    assert np.array_equal(res[:4], t)
+    assert np.array_equal(res[4], fin)
