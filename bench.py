@@ -74,6 +74,56 @@ def time_wf(qcb, ctx, L, prec, steps, warmup, eps=0.02, seed=11):
                 avg_stage_launch_us=launch_s * 1e6, achieved_gbs=achieved)
 
 
+def measured_traffic(kernel="k_hb_half<float"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary
+    (FETCH_SIZE and WRITE_SIZE scaled by the plain-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "hb_flow_32x4_rocprof.md")
+    if not os.path.exists(path):
+        return None
+    vals, section = [], False
+    for line in open(path):
+        if line.startswith("## "):
+            section = line.startswith("## HBM bytes")
+        if section and line.startswith("| `" + kernel):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            try:
+                vals.append(float(cells[3]) * 1e6)
+            except (ValueError, IndexError):
+                pass
+    return sum(vals) / len(vals) if vals else None
+
+
+def batch64(qcb, ctx, sweeps=20, flow_steps=50):
+    """64 lattices, 16 each of L in {8,10,12,16}, beta ~ U[5.8,6.2] (seed 3),
+    FP32 links: autotuned schedule, sweeps replayed from a hipGraph, then a
+    batched 50-step flow with a measurement per step."""
+    import numpy as np
+    rng = np.random.default_rng(3)
+    dims = [(L,) * 4 for L in (8, 10, 12, 16) for _ in range(16)]
+    betas = rng.uniform(5.8, 6.2, size=64)
+    b = qcb.Batch(ctx, dims, qcb.F32, betas, list(range(1, 65)))
+    b.hot()
+    report = b.autotune(2, 2)
+    order, grid = b.get_schedule()
+    b.graph_capture(2, 10, first_sweep=0)
+    b.graph_launch(1)
+    ctx.sync()
+    reps = max(1, sweeps // 10)
+    ctx.timing_begin()
+    b.graph_launch(reps)
+    ms = ctx.timing_end()
+    V = int(sum(np.prod(d) for d in dims))
+    t0 = time.perf_counter()
+    b.flow(0.02, flow_steps)
+    dt = time.perf_counter() - t0
+    P = b.plaquette()
+    b.close()
+    return dict(lattices=64, sites=V, link_updates_per_s=4 * V * 10 * reps / (ms * 1e-3),
+                ms_per_sweep=ms / (10 * reps), schedule={"order": order, "grid": grid},
+                autotune=report.strip().split("\n"),
+                flow_site_steps_per_s_incl_measure=V * flow_steps / dt, mean_plaquette=float(P.mean()))
+
+
 def serve(qcb, ctx, n_req=60, rate=20.0, n_therm=50, seed=2026):
     """Flow-measurement service under Poisson arrivals: L in {12,16,24},
     T = 2L, beta in {5.9,6.0,6.1}, t_max ~ U[0.5, 2.0], eps = 0.02, FP32 links
@@ -138,6 +188,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -162,6 +213,11 @@ def main():
         log(f"[bench] WF FP64 {wf64['site_steps_per_s']:.3e} site-steps/s, {wf64['achieved_gbs']:.0f} GB/s")
         comp.update(hb_fp64=hb64, wf_fp64=wf64)
         comp["fp32_over_fp64_hb"] = hb32["link_updates_per_s"] / hb64["link_updates_per_s"]
+    if not args.no_batch:
+        log("[bench] batch of 64 lattices (L in {8,10,12,16}), autotuned + hipGraph")
+        bt = batch64(qcb, ctx)
+        log(f"[bench] batch64 {bt['link_updates_per_s']:.3e} link-updates/s")
+        comp["batch64_fp32"] = bt
     if not args.no_serve:
         log(f"[bench] flow service: {args.serve_requests} Poisson requests at 20/s")
         sv = serve(qcb, ctx, args.serve_requests)
@@ -192,7 +248,7 @@ def main():
         "config": {"workload": f"heatbath sweep, {L}^4 periodic lattice, beta=6.0, N_hits=2, FP32 links, FP64 sums",
                    "lattice": [L, L, L, L], "sites": V, "beta": 6.0, "n_hits": 2},
         "roofline": {"bound": "hbm", "achieved": hb32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": hb32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": hb32["achieved_gbs"] / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "k_hb_half<float> (one (mu,parity) half-sweep)",
                      "bytes_per_launch": HB_BYTES_PER_LINK * 4 * (V // 2)},
         "cpu_baseline": cpu,

@@ -110,7 +110,7 @@ extern "C" int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb
     l->Vh = (uint32_t)(V / 2);
     l->S = (l->Vh + 63u) & ~63u;
     l->esize = prec == QCB_F64 ? 8 : 4;
-    default_tiling(dims, l->tile);
+    default_tiling(dims, l->tile, &l->morton);
     int rc = lattice_alloc(l);
     if (rc != QCB_OK) {
         lattice_free(l);
@@ -138,13 +138,18 @@ extern "C" int qcb_lattice_destroy(qcb_lattice* lat) {
     return QCB_OK;
 }
 
-extern "C" int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt) {
+extern "C" int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt, int morton) {
     if (!lat) return QCB_EINVAL;
     const int t[4] = {0, ty, tz, tt};
-    for (int d = 1; d < 4; ++d)
+    for (int d = 1; d < 4; ++d) {
         if (t[d] < 1 || lat->L[d] % t[d])
             return qcb_set_error(lat->ctx, QCB_EINVAL, "tile extent %d does not divide %d", t[d], lat->L[d]);
+        const int n = lat->L[d] / t[d];
+        if (morton && (n & (n - 1)))
+            return qcb_set_error(lat->ctx, QCB_EINVAL, "Z-order needs power-of-two tile counts (%d)", n);
+    }
     for (int d = 1; d < 4; ++d) lat->tile[d] = t[d];
+    lat->morton = morton ? 1 : 0;
     return upload_desc(lat);
 }
 

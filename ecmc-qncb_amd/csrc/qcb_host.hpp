@@ -39,6 +39,7 @@ struct qcb_lattice {
     bool flow_active = false;   // flow buffers hold a GradientFlow state (qcb_flow_copy)
     int tile[4] = {0, 1, 1, 1};  // work-order tile extents (tile[0] unused: whole x rows)
     uint32_t sweep_off = 0;      // per-lattice offset of the Philox sweep index
+    int morton = 0;              // Z-order over tiles (needs power-of-two tile counts)
 };
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
@@ -62,12 +63,26 @@ inline int default_tile(int L) {
     return 1;
 }
 
-// Default tiling: whole y rows (y up to 32), 4 x 4 in z and t when they divide.
-inline void default_tiling(const int L[4], int tile[4]) {
+// Largest divisor of L that is <= cap.
+inline int divisor_le(int L, int cap) {
+    for (int t = cap; t > 1; --t)
+        if (L % t == 0) return t;
+    return 1;
+}
+
+// Default tiling: 8 x 4 x 4 sites in (y, z, t) (largest divisors), tiles in
+// Z-order when every tile count is a power of two, else y-fastest.
+inline void default_tiling(const int L[4], int tile[4], int* morton) {
     tile[0] = 0;
-    tile[1] = L[1] <= 32 ? L[1] : default_tile(L[1]);
-    tile[2] = default_tile(L[2]);
-    tile[3] = default_tile(L[3]);
+    tile[1] = divisor_le(L[1], 8);
+    tile[2] = divisor_le(L[2], 4);
+    tile[3] = divisor_le(L[3], 4);
+    int m = 1;
+    for (int d = 1; d < 4; ++d) {
+        const int n = L[d] / tile[d];
+        if (n & (n - 1)) m = 0;
+    }
+    *morton = m;
 }
 
 // Blocks one lattice contributes to a launch of the given kernel family.
@@ -97,6 +112,13 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
         d.NT[k] = l->L[k] / l->tile[k];
     }
     d.tvol = (uint32_t)(l->L[0] / 2) * l->tile[1] * l->tile[2] * l->tile[3];
+    d.morton = l->morton;
+    d.tbits[0] = 0;
+    for (int k = 1; k < 4; ++k) {
+        int b = 0;
+        while ((1 << b) < d.NT[k]) ++b;
+        d.tbits[k] = b;
+    }
     return d;
 }
 

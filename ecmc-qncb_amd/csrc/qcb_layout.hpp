@@ -47,6 +47,8 @@ struct LatDev {
     int TL[4];            // tile extents in sites (TL[0] = Lx: whole x rows)
     int NT[4];            // tiles per direction (NT[0] = 1)
     uint32_t tvol;        // cb sites per tile = (Lx/2) * T1 * T2 * T3
+    int morton;           // tiles in Z-order over (ty, tz, tt) (NT powers of two) instead of y-fastest
+    int tbits[4];         // log2 NT[d] when morton
 };
 
 // Threads per block of each kernel family and the work item of a thread:
@@ -90,11 +92,24 @@ __device__ __forceinline__ Coords tiled_coords(const LatDev<T>& L, uint32_t q, i
     uint32_t r3 = r2 / (uint32_t)L.TL[2];
     int zl = (int)(r2 - r3 * (uint32_t)L.TL[2]);
     int tl = (int)r3;
-    uint32_t u1 = tau / (uint32_t)L.NT[1];
-    int ty = (int)(tau - u1 * (uint32_t)L.NT[1]);
-    uint32_t u2 = u1 / (uint32_t)L.NT[2];
-    int tz = (int)(u1 - u2 * (uint32_t)L.NT[2]);
-    int tt = (int)u2;
+    int ty, tz, tt;
+    if (L.morton) {
+        // bit b of each tile coordinate, y then z then t, for b = 0, 1, ...
+        ty = tz = tt = 0;
+        int pos = 0;
+        const int maxb = max(L.tbits[1], max(L.tbits[2], L.tbits[3]));
+        for (int b = 0; b < maxb; ++b) {
+            if (b < L.tbits[1]) ty |= (int)((tau >> pos++) & 1u) << b;
+            if (b < L.tbits[2]) tz |= (int)((tau >> pos++) & 1u) << b;
+            if (b < L.tbits[3]) tt |= (int)((tau >> pos++) & 1u) << b;
+        }
+    } else {
+        uint32_t u1 = tau / (uint32_t)L.NT[1];
+        ty = (int)(tau - u1 * (uint32_t)L.NT[1]);
+        uint32_t u2 = u1 / (uint32_t)L.NT[2];
+        tz = (int)(u1 - u2 * (uint32_t)L.NT[2]);
+        tt = (int)u2;
+    }
     Coords c;
     c.y = ty * L.TL[1] + yl;
     c.z = tz * L.TL[2] + zl;

@@ -248,7 +248,7 @@ extern "C" int qcb_service_submit(qcb_service* s, uint64_t id, const int dims[4]
     l.Vh = (uint32_t)(V / 2);
     l.S = (l.Vh + 63u) & ~63u;
     l.esize = s->prec == QCB_F64 ? 8 : 4;
-    default_tiling(dims, l.tile);   // the same tiling as a lattice created alone
+    default_tiling(dims, l.tile, &l.morton);   // the same tiling as a lattice created alone
     l.beta = beta;
     l.seed = seed;
     r->t_max = t_max;

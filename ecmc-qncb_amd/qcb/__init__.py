@@ -46,7 +46,7 @@ SIGNATURES = [
     ("qcb_lattice_create", C.c_int, [_VP, C.POINTER(C.c_int), C.c_int, C.POINTER(_VP)]),
     ("qcb_lattice_destroy", C.c_int, [_VP]),
     ("qcb_lattice_bytes", C.c_int, [_VP, C.POINTER(C.c_uint64)]),
-    ("qcb_lattice_set_tiling", C.c_int, [_VP, C.c_int, C.c_int, C.c_int]),
+    ("qcb_lattice_set_tiling", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("qcb_lattice_set_links", C.c_int, [_VP, _D]),
     ("qcb_lattice_get_links", C.c_int, [_VP, _D]),
     ("qcb_cold", C.c_int, [_VP]),
@@ -203,8 +203,8 @@ class Lattice:
         self._c(lib().qcb_lattice_get_links(self.h, _dp(out)))
         return out.view(np.complex128).reshape(self.V, 4, 3, 3)
 
-    def set_tiling(self, ty: int, tz: int, tt: int):
-        self._c(lib().qcb_lattice_set_tiling(self.h, ty, tz, tt))
+    def set_tiling(self, ty: int, tz: int, tt: int, morton: bool = False):
+        self._c(lib().qcb_lattice_set_tiling(self.h, ty, tz, tt, int(morton)))
 
     def nbytes(self) -> int:
         b = C.c_uint64()

@@ -65,11 +65,13 @@ int qcb_lattice_destroy(qcb_lattice* lat);
 /* Device bytes held by the handle (links plus flow buffers once allocated). */
 int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes);
 /* Work-order tiling of the neighbour-reading kernels: tiles of Lx x ty x tz x tt
- * sites (each must divide its extent).  Changes speed only: every kernel
+ * sites (each must divide its extent), visited y-fastest or, with morton != 0,
+ * in Z-order over the tile grid (power-of-two tile counts only).  Changes
+ * speed only: every kernel
  * result is independent of it except the last bits of the FP64 global sums,
- * whose block partition follows the tiles.  Default: whole y rows (Ly <= 32),
- * 4 x 4 in z and t (largest divisors <= 4). */
-int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt);
+ * whose block partition follows the tiles.  Default: 8 x 4 x 4 (largest
+ * divisors), Z-order when all tile counts are powers of two. */
+int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt, int morton);
 /* Upload / download of the whole field (src/io/ildg.cpp:87-106,213-232 minus
  * the byte swap, which stays with the caller's file codec). */
 int qcb_lattice_set_links(qcb_lattice* lat, const double* aos);
