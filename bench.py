@@ -162,6 +162,67 @@ def serve(qcb, ctx, n_req=60, rate=20.0, n_therm=50, seed=2026):
                 mean_sites=sites / n_req)
 
 
+def precision_report(qcb, ctx, L=32, sweeps=500, flow_steps=150, seed=11, beta=6.0):
+    """FP32 links (reunitarised every 4 sweeps) against the FP64 GPU run on
+    identical Philox uniforms: |<P>| differences along the chain, then the
+    Wilson flow of the final configurations.  'flow_only' flows the FP64
+    final configuration in both precisions, isolating the flow's rounding
+    from the divergence of the two Markov chains."""
+    import numpy as np
+    a = qcb.Lattice(ctx, (L,) * 4, qcb.F64)
+    b = qcb.Lattice(ctx, (L,) * 4, qcb.F32)
+    for x in (a, b):
+        x.hot(seed)
+    marks = {1, 2, 5, 10, 20, 50, 100, 200, 300, 400, 500}
+    chain, pa_eq, pb_eq = [], [], []
+    n_eq0 = min(100, sweeps // 5)
+    for s_ in range(sweeps):
+        a.hb_sweeps(beta, 2, 1, seed=seed, sweep_idx=s_)
+        b.hb_sweeps(beta, 2, 1, seed=seed, sweep_idx=s_)
+        if (s_ + 1) % 4 == 0:
+            b.reunitarize()
+        if s_ + 1 in marks or s_ >= n_eq0:
+            pa, pb = a.plaquette(), b.plaquette()
+            if s_ + 1 in marks:
+                chain.append(dict(sweep=s_ + 1, P64=pa, P32=pb, dP=abs(pa - pb)))
+            if s_ >= n_eq0:
+                pa_eq.append(pa)
+                pb_eq.append(pb)
+
+    def mean_err(x, nbin=20):
+        x = np.asarray(x)
+        bins = x[: len(x) // nbin * nbin].reshape(nbin, -1).mean(1)
+        return float(bins.mean()), float(bins.std(ddof=1) / np.sqrt(nbin))
+
+    m64, e64 = mean_err(pa_eq)
+    m32, e32 = mean_err(pb_eq)
+    equilibrium = dict(sweeps=f"{n_eq0 + 1}..{sweeps}", P64=m64, err64=e64, P32=m32, err32=e32,
+                       diff=m32 - m64, sigma=float(np.hypot(e64, e32)))
+    # the same configuration measured in both precisions
+    c0 = qcb.Lattice(ctx, (L,) * 4, qcb.F32)
+    c0.set_links(a.get_links())
+    p_meas = dict(P64=a.plaquette(), P32_same_links=c0.plaquette())
+    p_meas["dP"] = abs(p_meas["P64"] - p_meas["P32_same_links"])
+    c0.close()
+    ta = a.topo_charge_flowed(0.02, flow_steps + 1, 1)
+    tb = b.topo_charge_flowed(0.02, flow_steps + 1, 1)
+    c = qcb.Lattice(ctx, (L,) * 4, qcb.F32)
+    c.set_links(a.get_links())
+    tc = c.topo_charge_flowed(0.02, flow_steps + 1, 1)
+    rows = []
+    for t in (0.5, 1.0, 2.0, 3.0):
+        i = int(round(t / 0.02))
+        if i >= len(ta):
+            continue
+        rows.append(dict(t=t, Q64=ta[i, 1], E64=ta[i, 2], dQ_chain=abs(tb[i, 1] - ta[i, 1]),
+                         dE_rel_chain=abs(tb[i, 2] - ta[i, 2]) / ta[i, 2],
+                         dQ_flow_only=abs(tc[i, 1] - ta[i, 1]), dE_rel_flow_only=abs(tc[i, 2] - ta[i, 2]) / ta[i, 2]))
+    for x in (a, b, c):
+        x.close()
+    return dict(lattice=[L] * 4, beta=beta, sweeps=sweeps, flow_steps=flow_steps, chain=chain,
+                equilibrium=equilibrium, plaquette_measurement=p_meas, flow=rows)
+
+
 def cpu_baseline(L=32):
     """One reference-order FP64 sweep (N_hits = 2, beta = 6.0) of a hot L^4
     lattice on one core; counts only the non-frozen links it updates."""
@@ -182,13 +243,15 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--flow-steps", type=int, default=10)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="run the FP32-vs-FP64 comparison (32^4, 500 sweeps, 150 flow steps) and print it")
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -199,6 +262,10 @@ def main():
     import qcb
     ctx = qcb.Context(0)
     L = args.L
+    if args.precision_report:
+        print(json.dumps(precision_report(qcb, ctx, L)))
+        ctx.close()
+        return
     log(f"[bench] HB FP32 {L}^4: warmup {args.warmup}, timed {args.steps} sweeps")
     hb32 = time_hb(qcb, ctx, L, qcb.F32, args.steps, args.warmup)
     log(f"[bench] HB FP32 {hb32['link_updates_per_s']:.3e} link-updates/s, {hb32['achieved_gbs']:.0f} GB/s")

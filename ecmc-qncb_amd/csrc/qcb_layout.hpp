@@ -145,24 +145,41 @@ __device__ __forceinline__ size_t link_base(uint32_t S, int hs, uint32_t i) {
     return ((size_t)hs * (S >> 6) + (i >> 6)) * 576u + (i & 63u);
 }
 
+// Field pointers live in descriptors (generic pointers); casting them to the
+// global address space lets the compiler emit global_load/global_store
+// instead of flat accesses, whose waits also block on LDS traffic.
+#define QCB_GLOBAL __attribute__((address_space(1)))
+
+template <class T> struct Vec2;
+template <> struct Vec2<float> { typedef float type __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
 template <class T>
 __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, uint32_t i) {
-    const Cplx<T>* b = reinterpret_cast<const Cplx<T>*>(f) + link_base(S, mu * 2 + p, i);
+    typedef typename Vec2<T>::type V2;
+    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + link_base(S, mu * 2 + p, i);
     M3<T> m;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        Cplx<T> c = b[64 * k];
-        m.v[2 * k] = c.re;
-        m.v[2 * k + 1] = c.im;
+        V2 c = b[64 * k];
+        m.v[2 * k] = c.x;
+        m.v[2 * k + 1] = c.y;
     }
     return m;
 }
 
+// Every kernel writes whole fields that no thread of the same launch reads
+// back, so stores are non-temporal: they stream through L2 without evicting
+// the neighbour links the staples are still reusing.
 template <class T>
 __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_t i, const M3<T>& m) {
-    Cplx<T>* b = reinterpret_cast<Cplx<T>*>(f) + link_base(S, mu * 2 + p, i);
+    typedef typename Vec2<T>::type V2;
+    QCB_GLOBAL V2* b = (QCB_GLOBAL V2*)f + link_base(S, mu * 2 + p, i);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) b[64 * k] = Cplx<T>{m.v[2 * k], m.v[2 * k + 1]};
+    for (int k = 0; k < 9; ++k) {
+        V2 v = {m.v[2 * k], m.v[2 * k + 1]};
+        __builtin_nontemporal_store(v, b + 64 * k);
+    }
 }
 
 // Block index -> work tile such that each XCD owns one contiguous range of
