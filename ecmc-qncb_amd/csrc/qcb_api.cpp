@@ -484,11 +484,14 @@ extern "C" int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_
 // ---------------------------------------------------------- math probes ---
 
 extern "C" int qcb_math_probe(qcb_ctx* ctx, int prec, int op, int n, const double* in, double param, int aux, double* out) {
-    if (!ctx || !in || !out || n < 0 || op < 0 || op > 4) return QCB_EINVAL;
+    if (!ctx || !in || !out || n < 0 || op < 0 || op > 5) return QCB_EINVAL;
     if ((op == 3 || op == 4) && (aux < 1 || aux > 64)) return QCB_EINVAL;
+    if (op == 5)
+        for (int t = 0; t < n; ++t)
+            if (!(in[t] >= 0.0 && in[t] <= 4294967295.0 && in[t] == (double)(uint64_t)in[t])) return QCB_EINVAL;
     if (n == 0) return QCB_OK;
-    size_t win = op <= 2 ? 18 : op == 3 ? (size_t)(1 + 4 * aux + 2) : (size_t)(36 + 3 * (4 * aux + 2));
-    size_t wout = op == 3 ? 5 : 18;
+    size_t win = op <= 2 ? 18 : op == 3 ? (size_t)(1 + 4 * aux + 2) : op == 4 ? (size_t)(36 + 3 * (4 * aux + 2)) : 1;
+    size_t wout = op == 3 ? 5 : op == 5 ? 1 : 18;
     double *di = nullptr, *dout = nullptr;
     QCB_HIP(ctx, hipMallocAsync((void**)&di, win * n * sizeof(double), ctx->stream));
     QCB_HIP(ctx, hipMallocAsync((void**)&dout, wout * n * sizeof(double), ctx->stream));

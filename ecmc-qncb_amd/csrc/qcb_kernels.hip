@@ -454,13 +454,31 @@ struct KpMath {
         *c = __builtin_amdgcn_cosf(u);
     }
     __device__ static __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // 1/x, alpha = 2 beta k / 3 and -1/(2 alpha): correctly rounded divisions
+    // in double (the oracle's arithmetic), v_rcp_f32 (1 ulp) in float
+    __device__ static __forceinline__ double rcp(double x) { return 1.0 / x; }
+    __device__ static __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    __device__ static __forceinline__ double alpha(double beta, double k) { return beta * k * 2.0 / 3.0; }
+    __device__ static __forceinline__ float alpha(float beta, float k) { return beta * k * (2.0f / 3.0f); }
+    __device__ static __forceinline__ double neg_half_over(double a) { return -0.5 / a; }
+    __device__ static __forceinline__ float neg_half_over(float a) { return -0.5f * __builtin_amdgcn_rcpf(a); }
+    // heatbath uniform of a Philox word: (w + 1/2) 2^-32 in the link precision
+    __device__ static __forceinline__ void uni(uint32_t w, double* u) { *u = u32d(w); }
+    __device__ static __forceinline__ void uni(uint32_t w, float* u) { *u = u32f(w); }
+    template <class T>
+    __device__ static __forceinline__ T uni(uint32_t w) {
+        T u;
+        uni(w, &u);
+        return u;
+    }
 };
 
-// One Kennedy-Pendleton attempt (reference src/heatbath/heatbath.cpp:14-26).
+// One Kennedy-Pendleton attempt (reference src/heatbath/heatbath.cpp:14-26);
+// nhia = -1/(2 alpha).
 template <class T>
-__device__ __forceinline__ bool kp_try(T alpha, T delta, T r1, T r2, T r3, T& a0) {
+__device__ __forceinline__ bool kp_try(T nhia, T delta, T r1, T r2, T r3, T& a0) {
     T cs = KpMath::cos2pi(r2);
-    T lam2 = T(-0.5) / alpha * (KpMath::ln(r1) + cs * cs * KpMath::ln(r3));
+    T lam2 = nhia * (KpMath::ln(r1) + cs * cs * KpMath::ln(r3));
     if (delta * delta <= T(1) - lam2) {
         a0 = T(1) - T(2) * lam2;
         return true;
@@ -519,9 +537,9 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
         T q0 = (w[0] + w[6]) * T(0.5), q1 = (w[3] + w[5]) * T(0.5);
         T q2 = (w[2] - w[4]) * T(0.5), q3 = (w[1] - w[7]) * T(0.5);
         T k = KpMath::sq(q0 * q0 + q1 * q1 + q2 * q2 + q3 * q3);
-        T ik = T(1) / k;
+        T ik = KpMath::rcp(k);
         T v0 = q0 * ik, v1 = q1 * ik, v2 = q2 * ik, v3 = q3 * ik;
-        T alpha = beta * k * T(2) / T(3);
+        const T nhia = KpMath::neg_half_over(KpMath::alpha(beta, k));
         T x[4];
         bool ok = false;
         if (MODE == kHbInjected) {
@@ -529,7 +547,7 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
             const double* u = inj + ((size_t)((hs * n_hits + h) * 3 + g) * (size_t)nslot) * Vh + cb;
             T a0 = T(0);
             for (int a = 0; a < K; ++a) {
-                if (kp_try<T>(alpha, (T)u[(size_t)(4 * a) * Vh], (T)u[(size_t)(4 * a + 1) * Vh],
+                if (kp_try<T>(nhia, (T)u[(size_t)(4 * a) * Vh], (T)u[(size_t)(4 * a + 1) * Vh],
                               (T)u[(size_t)(4 * a + 2) * Vh], (T)u[(size_t)(4 * a + 3) * Vh], a0)) {
                     kp_sphere<T>(a0, (T)u[(size_t)(4 * K) * Vh], (T)u[(size_t)(4 * K + 1) * Vh], x);
                     ok = true;
@@ -541,9 +559,10 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
             T a0 = T(0);
             for (uint32_t a = 0; a < kPhiloxMaxAttempts; ++a) {
                 U4 o = philox4x32_10(U4{cb, sweep, cz, a}, key0, key1);
-                if (kp_try<T>(alpha, (T)u32d(o.x), (T)u32d(o.y), (T)u32d(o.z), (T)u32d(o.w), a0)) {
+                if (kp_try<T>(nhia, KpMath::uni<T>(o.x), KpMath::uni<T>(o.y), KpMath::uni<T>(o.z),
+                              KpMath::uni<T>(o.w), a0)) {
                     U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
-                    kp_sphere<T>(a0, (T)u32d(os.x), (T)u32d(os.y), x);
+                    kp_sphere<T>(a0, KpMath::uni<T>(os.x), KpMath::uni<T>(os.y), x);
                     ok = true;
                     break;
                 }
@@ -639,33 +658,27 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 //  stage 1: Zs = Z,                      W' = exp(eps/4 Z) W
 //  stage 2: Zs = 8/9 Z - 17/36 Zs,       W' = exp(eps Zs) W
 //  stage 3:                              W' = exp(eps (3/4 Z - Zs)) W
+// Z and Zs are carried as their 9 independent reals (A9).
 template <class T, int MU, int STAGE>
 __device__ __forceinline__ void flow_link(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
                                           T* __restrict__ zs, Coords c, int p, uint32_t i, T eps) {
     M3<T> Sg = staple<T, MU>(L, win, c, p, i);
     M3<T> W = ld_link(win, L.S, MU, p, i);
-    M3<T> Z = mul_nn(W, Sg);
-#pragma unroll
-    for (int k = 0; k < 18; ++k) Z.v[k] = -Z.v[k];
-    proj_lie(Z);
-    M3<T> X;
+    A9<T> Z = force_ta(W, Sg);
+    A9<T> X;
     if (STAGE == 1) {
-        st_link(zs, L.S, MU, p, i, Z);
-#pragma unroll
-        for (int k = 0; k < 18; ++k) X.v[k] = T(0.25) * eps * Z.v[k];
+        st_a9(zs, L.S, MU, p, i, Z);
+        X = a9_scale(T(0.25) * eps, Z);
     } else if (STAGE == 2) {
-        M3<T> Z0 = ld_link((const T*)zs, L.S, MU, p, i);
-#pragma unroll
-        for (int k = 0; k < 18; ++k) Z0.v[k] = T(8.0 / 9.0) * Z.v[k] - T(17.0 / 36.0) * Z0.v[k];
-        st_link(zs, L.S, MU, p, i, Z0);
-#pragma unroll
-        for (int k = 0; k < 18; ++k) X.v[k] = eps * Z0.v[k];
+        A9<T> Z0 = ld_a9((const T*)zs, L.S, MU, p, i);
+        A9<T> Z1 = a9_axpby(T(8.0 / 9.0), Z, T(-17.0 / 36.0), Z0);
+        st_a9(zs, L.S, MU, p, i, Z1);
+        X = a9_scale(eps, Z1);
     } else {
-        M3<T> Zc = ld_link((const T*)zs, L.S, MU, p, i);
-#pragma unroll
-        for (int k = 0; k < 18; ++k) X.v[k] = T(0.75) * eps * Z.v[k] - eps * Zc.v[k];
+        A9<T> Zc = ld_a9((const T*)zs, L.S, MU, p, i);
+        X = a9_axpby(T(0.75) * eps, Z, -eps, Zc);
     }
-    M3<T> E = exp_su3<T, ExpTerms<T>::N>(X);
+    M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
     st_link(wout, L.S, MU, p, i, mul_nn(E, W));
 }
 
@@ -703,8 +716,14 @@ __global__ void k_math_probe(int op, int n, const double* __restrict__ in, doubl
         M3<T> a;
         for (int k = 0; k < 18; ++k) a.v[k] = (T)in[(size_t)t * 18 + k];
         if (op == 0) {
-            for (int k = 0; k < 18; ++k) a.v[k] = (T)param * a.v[k];
-            a = exp_su3<T, ExpTerms<T>::N>(a);
+            // the flow's exponential: only the upper triangle and the imaginary
+            // diagonal of the (anti-Hermitian) input are read
+            A9<T> z;
+            z.o[0] = a.re(0, 1); z.o[1] = a.im(0, 1);
+            z.o[2] = a.re(0, 2); z.o[3] = a.im(0, 2);
+            z.o[4] = a.re(1, 2); z.o[5] = a.im(1, 2);
+            for (int k = 0; k < 3; ++k) z.d[k] = a.im(k, k);
+            a = exp_su3_ah<T, ExpTerms<T>::N>(a9_scale((T)param, z));
         } else if (op == 1) {
             proj_lie(a);
         } else {
@@ -718,11 +737,11 @@ __global__ void k_math_probe(int op, int n, const double* __restrict__ in, doubl
         const int K = aux, w = 1 + 4 * K + 2;
         const double* u = in + (size_t)t * w;
         T k = (T)u[0];
-        T alpha = (T)param * k * T(2) / T(3);
+        const T nhia = KpMath::neg_half_over(KpMath::alpha((T)param, k));
         T a0 = T(0), x[4] = {T(0), T(0), T(0), T(0)};
         int acc = -1;
         for (int a = 0; a < K; ++a)
-            if (kp_try<T>(alpha, (T)u[1 + 4 * a], (T)u[2 + 4 * a], (T)u[3 + 4 * a], (T)u[4 + 4 * a], a0)) {
+            if (kp_try<T>(nhia, (T)u[1 + 4 * a], (T)u[2 + 4 * a], (T)u[3 + 4 * a], (T)u[4 + 4 * a], a0)) {
                 kp_sphere<T>(a0, (T)u[1 + 4 * K], (T)u[2 + 4 * K], x);
                 acc = a;
                 break;
@@ -737,6 +756,8 @@ __global__ void k_math_probe(int op, int n, const double* __restrict__ in, doubl
         // uniforms laid out [g][slot] with stride 1: reuse the injected path with Vh = 1
         cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1);
         for (int k = 0; k < 18; ++k) out[(size_t)t * 18 + k] = (double)U.v[k];
+    } else if (op == 5) {
+        out[t] = (double)KpMath::uni<T>((uint32_t)in[t]);
     }
 }
 

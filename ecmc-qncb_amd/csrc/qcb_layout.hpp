@@ -182,6 +182,44 @@ __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_
     }
 }
 
+// Force fields hold traceless anti-Hermitian matrices as 5 complex slots per
+// link (A9 in qcb_su3.hpp), in the same AoSoA-64 order as the links.
+__device__ __forceinline__ size_t a9_base(uint32_t S, int hs, uint32_t i) {
+    return ((size_t)hs * (S >> 6) + (i >> 6)) * 320u + (i & 63u);
+}
+
+template <class T>
+__device__ __forceinline__ A9<T> ld_a9(const T* f, uint32_t S, int mu, int p, uint32_t i) {
+    typedef typename Vec2<T>::type V2;
+    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + a9_base(S, mu * 2 + p, i);
+    A9<T> z;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        V2 c = b[64 * k];
+        z.o[2 * k] = c.x;
+        z.o[2 * k + 1] = c.y;
+    }
+    V2 c3 = b[64 * 3], c4 = b[64 * 4];
+    z.d[0] = c3.x;
+    z.d[1] = c3.y;
+    z.d[2] = c4.x;
+    return z;
+}
+
+template <class T>
+__device__ __forceinline__ void st_a9(T* f, uint32_t S, int mu, int p, uint32_t i, const A9<T>& z) {
+    typedef typename Vec2<T>::type V2;
+    QCB_GLOBAL V2* b = (QCB_GLOBAL V2*)f + a9_base(S, mu * 2 + p, i);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        V2 v = {z.o[2 * k], z.o[2 * k + 1]};
+        __builtin_nontemporal_store(v, b + 64 * k);
+    }
+    V2 v3 = {z.d[0], z.d[1]}, v4 = {z.d[2], T(0)};
+    __builtin_nontemporal_store(v3, b + 64 * 3);
+    __builtin_nontemporal_store(v4, b + 64 * 4);
+}
+
 // Block index -> work tile such that each XCD owns one contiguous range of
 // tiles (blocks are dispatched to XCD b % 8), keeping neighbour rows of the
 // lattice in the same L2.

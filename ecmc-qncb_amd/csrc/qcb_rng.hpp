@@ -47,6 +47,14 @@ __device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
 
 __device__ __forceinline__ double u32d(uint32_t w) { return ((double)w + 0.5) * 0x1.0p-32; }
 
+// The float nearest to (w + 1/2) 2^-32, i.e. (float)u32d(w), without double
+// arithmetic: (w >> 8) 2^-24 and ((w & 255) + 1/2) 2^-32 are exact floats and
+// the fma rounds their exact sum once.
+__device__ __forceinline__ float u32f(uint32_t w) {
+    const float lo = __builtin_fmaf((float)(w & 255u), 0x1.0p-32f, 0x1.0p-33f);
+    return __builtin_fmaf((float)(w >> 8), 0x1.0p-24f, lo);
+}
+
 __host__ __device__ __forceinline__ uint32_t hb_ctr_z(int hs, int hit, int g) {
     return ((uint32_t)hs << 24) | ((uint32_t)hit << 8) | ((uint32_t)g << 4);
 }

@@ -187,15 +187,105 @@ __device__ __forceinline__ void proj_lie(M3<T>& a) {
             b.re(i, j) = T(0.5) * (a.re(i, j) - a.re(j, i));
             b.im(i, j) = T(0.5) * (a.im(i, j) + a.im(j, i));
         }
-    T tr = (b.im(0, 0) + b.im(1, 1) + b.im(2, 2)) / T(3);
-    // real parts of the diagonal of (A - A^dag)/2 are exactly zero
-    T trr = (b.re(0, 0) + b.re(1, 1) + b.re(2, 2)) / T(3);
+    T tr = (b.im(0, 0) + b.im(1, 1) + b.im(2, 2)) * T(1.0 / 3.0);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        b.re(i, i) -= trr;
+        b.re(i, i) = T(0);  // real part of the diagonal of (A - A^dag)/2
         b.im(i, i) -= tr;
     }
     a = b;
+}
+
+// A traceless anti-Hermitian matrix by its 9 independent reals (the
+// reference keeps all 18, src/flow/gradient_flow.cpp:15-34):
+//   o = {Re Z01, Im Z01, Re Z02, Im Z02, Re Z12, Im Z12},  d = {Im Z00, Im Z11, Im Z22},
+// Z10 = -conj(Z01) etc., Re Z_ii = 0.  Memory form: 5 complex slots
+// {Z01, Z02, Z12, (d0, d1), (d2, 0)}.
+template <class T>
+struct A9 {
+    T o[6];
+    T d[3];
+};
+
+// P_TA(-W S): only the entries of W S that the projection keeps are formed
+// -- both off-diagonal triangles and the imaginary diagonal.
+template <class T>
+__device__ __forceinline__ A9<T> force_ta(const M3<T>& W, const M3<T>& S) {
+    A9<T> z;
+    constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int i = pi[e], j = pj[e];
+        T rij = T(0), sij = T(0), rji = T(0), sji = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rij = fma(W.re(i, k), S.re(k, j), rij);
+            rij = fma(-W.im(i, k), S.im(k, j), rij);
+            sij = fma(W.re(i, k), S.im(k, j), sij);
+            sij = fma(W.im(i, k), S.re(k, j), sij);
+            rji = fma(W.re(j, k), S.re(k, i), rji);
+            rji = fma(-W.im(j, k), S.im(k, i), rji);
+            sji = fma(W.re(j, k), S.im(k, i), sji);
+            sji = fma(W.im(j, k), S.re(k, i), sji);
+        }
+        z.o[2 * e] = T(-0.5) * (rij - rji);
+        z.o[2 * e + 1] = T(-0.5) * (sij + sji);
+    }
+    T m[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            s = fma(W.re(i, k), S.im(k, i), s);
+            s = fma(W.im(i, k), S.re(k, i), s);
+        }
+        m[i] = s;
+    }
+    const T tr = (m[0] + m[1] + m[2]) * T(1.0 / 3.0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) z.d[i] = tr - m[i];
+    return z;
+}
+
+// z = a x + b y
+template <class T>
+__device__ __forceinline__ A9<T> a9_axpby(T a, const A9<T>& x, T b, const A9<T>& y) {
+    A9<T> z;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) z.o[k] = fma(a, x.o[k], b * y.o[k]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) z.d[k] = fma(a, x.d[k], b * y.d[k]);
+    return z;
+}
+
+template <class T>
+__device__ __forceinline__ A9<T> a9_scale(T a, const A9<T>& x) {
+    A9<T> z;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) z.o[k] = a * x.o[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) z.d[k] = a * x.d[k];
+    return z;
+}
+
+template <class T>
+__device__ __forceinline__ M3<T> a9_full(const A9<T>& x) {
+    M3<T> m;
+    constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        m.re(pi[e], pj[e]) = x.o[2 * e];
+        m.im(pi[e], pj[e]) = x.o[2 * e + 1];
+        m.re(pj[e], pi[e]) = -x.o[2 * e];
+        m.im(pj[e], pi[e]) = x.o[2 * e + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        m.re(i, i) = T(0);
+        m.im(i, i) = x.d[i];
+    }
+    return m;
 }
 
 // Taylor coefficients 1/n! built by successive division, as the reference does.
@@ -212,24 +302,56 @@ struct InvFact {
     }
 };
 
-// exp(X) for X in su(3) by Luescher's Cayley-Hamilton Horner scheme
-// (reference src/su3/utils.cpp:145-193): with t = -Re Tr X^2/2 and
-// d = Re(i det X), q(n) = c_n - i d q2, q1 <- q0 - t q2, q2 <- q1 from n = N-1
-// down to 0, and exp X = q0 + q1 X + q2 X^2.  N = 18 in double; float needs
-// fewer terms for the same relative accuracy (||X|| <= 1 in the flow).
+// exp(X) for X in su(3), given by its 9 reals, by Luescher's Cayley-Hamilton
+// Horner scheme (reference src/su3/utils.cpp:145-193): with t = -Re Tr X^2/2
+// and d = Re(i det X), q(n) = c_n - i d q2, q1 <- q0 - t q2, q2 <- q1 from
+// n = N-1 down to 0, and exp X = q0 + q1 X + q2 X^2.  N = 18 in double; float
+// needs fewer terms for the same relative accuracy (||X|| <= 1 in the flow).
+// X^2 is Hermitian, so only its upper triangle and real diagonal are
+// multiplied out, and the zero real diagonal of X is never multiplied.
 template <class T, int N>
-__device__ __forceinline__ M3<T> exp_su3(const M3<T>& X) {
-    M3<T> X2 = mul_nn(X, X);
-    T t = T(-0.5) * retrace(X2);
-    // Im det X by cofactors along row 0
-    T c0r = X.re(1, 1) * X.re(2, 2) - X.im(1, 1) * X.im(2, 2) - (X.re(1, 2) * X.re(2, 1) - X.im(1, 2) * X.im(2, 1));
-    T c0i = X.re(1, 1) * X.im(2, 2) + X.im(1, 1) * X.re(2, 2) - (X.re(1, 2) * X.im(2, 1) + X.im(1, 2) * X.re(2, 1));
-    T c1r = X.re(1, 2) * X.re(2, 0) - X.im(1, 2) * X.im(2, 0) - (X.re(1, 0) * X.re(2, 2) - X.im(1, 0) * X.im(2, 2));
-    T c1i = X.re(1, 2) * X.im(2, 0) + X.im(1, 2) * X.re(2, 0) - (X.re(1, 0) * X.im(2, 2) + X.im(1, 0) * X.re(2, 2));
-    T c2r = X.re(1, 0) * X.re(2, 1) - X.im(1, 0) * X.im(2, 1) - (X.re(1, 1) * X.re(2, 0) - X.im(1, 1) * X.im(2, 0));
-    T c2i = X.re(1, 0) * X.im(2, 1) + X.im(1, 0) * X.re(2, 1) - (X.re(1, 1) * X.im(2, 0) + X.im(1, 1) * X.re(2, 0));
-    T det_im = X.re(0, 0) * c0i + X.im(0, 0) * c0r + X.re(0, 1) * c1i + X.im(0, 1) * c1r + X.re(0, 2) * c2i +
-               X.im(0, 2) * c2r;
+__device__ __forceinline__ M3<T> exp_su3_ah(const A9<T>& x) {
+    const M3<T> X = a9_full(x);
+    M3<T> Y;  // X^2
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {  // Y_ii = -sum_k |X_ik|^2
+        T s = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k != i) s = fma(X.re(i, k), X.re(i, k), s);
+            s = fma(X.im(i, k), X.im(i, k), s);
+        }
+        Y.re(i, i) = -s;
+        Y.im(i, i) = T(0);
+    }
+    constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {  // Y_ij = sum_k X_ik X_kj, Y_ji = conj(Y_ij)
+        const int i = pi[e], j = pj[e];
+        T r = T(0), s = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const bool az = k == i, cz = k == j;  // Re X_ii = 0, Re X_jj = 0
+            if (!az && !cz) r = fma(X.re(i, k), X.re(k, j), r);
+            r = fma(-X.im(i, k), X.im(k, j), r);
+            if (!az) s = fma(X.re(i, k), X.im(k, j), s);
+            if (!cz) s = fma(X.im(i, k), X.re(k, j), s);
+        }
+        Y.re(i, j) = r;
+        Y.im(i, j) = s;
+        Y.re(j, i) = r;
+        Y.im(j, i) = -s;
+    }
+    const T t = T(-0.5) * (Y.re(0, 0) + Y.re(1, 1) + Y.re(2, 2));
+    // d = -Im det X by cofactors along row 0, with Re X_ii = 0 (so the
+    // imaginary part of the first cofactor is never needed)
+    const T x00 = X.im(0, 0), x11 = X.im(1, 1), x22 = X.im(2, 2);
+    T c0r = -x11 * x22 - (X.re(1, 2) * X.re(2, 1) - X.im(1, 2) * X.im(2, 1));
+    T c1r = X.re(1, 2) * X.re(2, 0) - X.im(1, 2) * X.im(2, 0) + X.im(1, 0) * x22;
+    T c1i = X.re(1, 2) * X.im(2, 0) + X.im(1, 2) * X.re(2, 0) - X.re(1, 0) * x22;
+    T c2r = X.re(1, 0) * X.re(2, 1) - X.im(1, 0) * X.im(2, 1) + x11 * X.im(2, 0);
+    T c2i = X.re(1, 0) * X.im(2, 1) + X.im(1, 0) * X.re(2, 1) - x11 * X.re(2, 0);
+    T det_im = x00 * c0r + X.re(0, 1) * c1i + X.im(0, 1) * c1r + X.re(0, 2) * c2i + X.im(0, 2) * c2r;
     T d = -det_im;
     constexpr InvFact<T, N> cf;
     T q0r = cf.c[N], q0i = T(0), q1r = T(0), q1i = T(0), q2r = T(0), q2i = T(0);
@@ -246,18 +368,20 @@ __device__ __forceinline__ M3<T> exp_su3(const M3<T>& X) {
         q0r = n0r;
         q0i = n0i;
     }
-    M3<T> e;
+    M3<T> e;  // q0 + q1 X + q2 Y
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        T xr = X.v[2 * k], xi = X.v[2 * k + 1], yr = X2.v[2 * k], yi = X2.v[2 * k + 1];
-        T r = q1r * xr - q1i * xi + (q2r * yr - q2i * yi);
-        T s = q1r * xi + q1i * xr + (q2r * yi + q2i * yr);
-        e.v[2 * k] = r;
-        e.v[2 * k + 1] = s;
-    }
-    e.v[0] += q0r; e.v[1] += q0i;
-    e.v[8] += q0r; e.v[9] += q0i;
-    e.v[16] += q0r; e.v[17] += q0i;
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T xr = X.re(i, j), xi = X.im(i, j), yr = Y.re(i, j), yi = Y.im(i, j);
+            if (i == j) {  // X_ii = i xi, Y_ii = yr
+                e.re(i, i) = fma(q2r, yr, q0r) - q1i * xi;
+                e.im(i, i) = fma(q2i, yr, q0i) + q1r * xi;
+            } else {
+                e.re(i, j) = q1r * xr - q1i * xi + (q2r * yr - q2i * yi);
+                e.im(i, j) = q1r * xi + q1i * xr + (q2r * yi + q2i * yr);
+            }
+        }
     return e;
 }
 

@@ -21,7 +21,7 @@ LIB_PATH = os.environ.get("QCB_LIBRARY") or os.path.join(PKG_DIR, "libqcb.so")
 
 F64, F32 = 0, 1
 OK, EINVAL, EDEVICE, ENOMEM, EBUDGET, ESTATE = 0, -1, -2, -3, -4, -5
-PROBE_EXP, PROBE_PROJ, PROBE_REUNIT, PROBE_KP, PROBE_CM_HIT = 0, 1, 2, 3, 4
+PROBE_EXP, PROBE_PROJ, PROBE_REUNIT, PROBE_KP, PROBE_CM_HIT, PROBE_UNIFORM = 0, 1, 2, 3, 4, 5
 
 _D = C.POINTER(C.c_double)
 _VP = C.c_void_p
@@ -157,7 +157,7 @@ class Context:
     def probe(self, prec: int, op: int, inp: np.ndarray, param: float = 0.0, aux: int = 0) -> np.ndarray:
         inp = np.ascontiguousarray(inp, dtype=np.float64)
         n = inp.shape[0]
-        wout = 5 if op == PROBE_KP else 18
+        wout = 5 if op == PROBE_KP else 1 if op == PROBE_UNIFORM else 18
         out = np.zeros((n, wout))
         self.check(lib().qcb_math_probe(self.h, prec, op, n, _dp(inp.reshape(n, -1)), param, aux, _dp(out)))
         return out

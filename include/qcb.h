@@ -207,13 +207,16 @@ int qcb_ildg_save(qcb_lattice* lat, const char* path);
 int qcb_ildg_load(qcb_lattice* lat, const char* path);
 
 /* ---- device math probes (checking aids for the register-level routines) ---- */
-#define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out */
+#define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out; in is
+                                  traceless anti-Hermitian: its upper triangle and Im diagonal are read */
 #define QCB_PROBE_PROJ 1       /* out = P_TA(in) (utils.cpp:130-134) */
 #define QCB_PROBE_REUNIT 2     /* out = Gram-Schmidt(in) (GaugeField.cpp:36-58) */
 #define QCB_PROBE_KP 3         /* in = {k, u[4K+2]} per item, param = beta, K = (int)aux;
                                   out = {accepted attempt or -1, x0..x3} (heatbath.cpp:8-36) */
 #define QCB_PROBE_CM_HIT 4     /* in = {U 18, A 18, u[3*(4K+2)]}, param = beta, aux = K:
                                   out = U after one CM hit (heatbath_mpi.cpp:11-34) */
+#define QCB_PROBE_UNIFORM 5    /* in = {w} (a 32-bit Philox word as a double), 1 in / 1 out:
+                                  out = the heatbath uniform (w + 1/2) 2^-32 in the precision prec */
 int qcb_math_probe(qcb_ctx* ctx, int prec, int op, int n, const double* in, double param, int aux,
                    double* out);
 

@@ -129,6 +129,24 @@ def test_probe_kp_and_cm_hit(ctx):
             assert np.abs(o[1:] - x).max() < 1e-14
 
 
+def test_probe_uniform_words(ctx):
+    """The heatbath's uniform of a Philox word is (w + 1/2) 2^-32 exactly in
+    double, and the float nearest to it in float (computed without double
+    arithmetic on the device)."""
+    rng = np.random.default_rng(9)
+    edge = [0, 1, 2, 127, 128, 255, 256, 257, 2**23 - 1, 2**23, 2**23 + 1, 2**24 - 1, 2**24, 2**24 + 1,
+            2**25 - 1, 2**25, 2**25 + 1, 2**31 - 1, 2**31, 2**32 - 2, 2**32 - 1]
+    w = np.concatenate([edge, rng.integers(0, 2**32, size=200000)]).astype(np.float64)
+    exact = (w + 0.5) * 2.0 ** -32
+    out64 = ctx.probe(qcb.F64, qcb.PROBE_UNIFORM, w[:, None])[:, 0]
+    assert np.array_equal(out64, exact)
+    out32 = ctx.probe(qcb.F32, qcb.PROBE_UNIFORM, w[:, None])[:, 0]
+    assert np.array_equal(out32, exact.astype(np.float32).astype(np.float64))
+    assert out32.min() > 0 and out32.max() <= 1.0
+    with pytest.raises(qcb.QcbError):
+        ctx.probe(qcb.F32, qcb.PROBE_UNIFORM, np.array([[2.0 ** 32]]))
+
+
 def _uniforms(rng, Vh, n_hits, K):
     return rng.uniform(size=8 * n_hits * 3 * (4 * K + 2) * Vh)
 
