@@ -280,6 +280,15 @@ def main():
         log(f"[bench] WF FP64 {wf64['site_steps_per_s']:.3e} site-steps/s, {wf64['achieved_gbs']:.0f} GB/s")
         comp.update(hb_fp64=hb64, wf_fp64=wf64)
         comp["fp32_over_fp64_hb"] = hb32["link_updates_per_s"] / hb64["link_updates_per_s"]
+        comp["fp32_over_fp64_wf"] = wf32["site_steps_per_s"] / wf64["site_steps_per_s"]
+        # the plain FP64 GPU configuration: 16^4, hot start from seed 7, N_hits = 2
+        hbp = time_hb(qcb, ctx, 16, qcb.F64, 40, 4, seed=7)
+        wfp = time_wf(qcb, ctx, 16, qcb.F64, 20, 2, seed=7)
+        log(f"[bench] plain FP64 16^4: HB {hbp['link_updates_per_s']:.3e} link-updates/s, "
+            f"WF {wfp['site_steps_per_s']:.3e} site-steps/s")
+        comp["plain_fp64_16x4"] = dict(hb=hbp, wf=wfp)
+        comp["fp32_32x4_over_plain_fp64_16x4_hb"] = hb32["link_updates_per_s"] / hbp["link_updates_per_s"]
+        comp["fp32_32x4_over_plain_fp64_16x4_wf"] = wf32["site_steps_per_s"] / wfp["site_steps_per_s"]
     if not args.no_batch:
         log("[bench] batch of 64 lattices (L in {8,10,12,16}), autotuned + hipGraph")
         bt = batch64(qcb, ctx)

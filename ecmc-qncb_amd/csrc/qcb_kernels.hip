@@ -540,29 +540,28 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
         T ik = KpMath::rcp(k);
         T v0 = q0 * ik, v1 = q1 * ik, v2 = q2 * ik, v3 = q3 * ik;
         const T nhia = KpMath::neg_half_over(KpMath::alpha(beta, k));
-        T x[4];
+        // Rejection loop of KP attempts only; the sphere draw follows once
+        // per subgroup, outside the loop, so a wavefront whose lanes accept
+        // at different attempts runs it once instead of once per attempt.
+        T a0 = T(0);
         bool ok = false;
+        const double* u = nullptr;
+        const uint32_t cz = hb_ctr_z(hs, h, g);
         if (MODE == kHbInjected) {
             const int nslot = 4 * K + 2;
-            const double* u = inj + ((size_t)((hs * n_hits + h) * 3 + g) * (size_t)nslot) * Vh + cb;
-            T a0 = T(0);
+            u = inj + ((size_t)((hs * n_hits + h) * 3 + g) * (size_t)nslot) * Vh + cb;
             for (int a = 0; a < K; ++a) {
                 if (kp_try<T>(nhia, (T)u[(size_t)(4 * a) * Vh], (T)u[(size_t)(4 * a + 1) * Vh],
                               (T)u[(size_t)(4 * a + 2) * Vh], (T)u[(size_t)(4 * a + 3) * Vh], a0)) {
-                    kp_sphere<T>(a0, (T)u[(size_t)(4 * K) * Vh], (T)u[(size_t)(4 * K + 1) * Vh], x);
                     ok = true;
                     break;
                 }
             }
         } else {
-            const uint32_t cz = hb_ctr_z(hs, h, g);
-            T a0 = T(0);
             for (uint32_t a = 0; a < kPhiloxMaxAttempts; ++a) {
                 U4 o = philox4x32_10(U4{cb, sweep, cz, a}, key0, key1);
                 if (kp_try<T>(nhia, KpMath::uni<T>(o.x), KpMath::uni<T>(o.y), KpMath::uni<T>(o.z),
                               KpMath::uni<T>(o.w), a0)) {
-                    U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
-                    kp_sphere<T>(a0, KpMath::uni<T>(os.x), KpMath::uni<T>(os.y), x);
                     ok = true;
                     break;
                 }
@@ -571,6 +570,13 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
         if (!ok) {
             ++exhausted;
             continue;
+        }
+        T x[4];
+        if (MODE == kHbInjected) {
+            kp_sphere<T>(a0, (T)u[(size_t)(4 * K) * Vh], (T)u[(size_t)(4 * K + 1) * Vh], x);
+        } else {
+            U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
+            kp_sphere<T>(a0, KpMath::uni<T>(os.x), KpMath::uni<T>(os.y), x);
         }
         // r = mult(x, adj(v)) (reference src/heatbath/heatbath.cpp:39-44, src/su3/utils.cpp:53-63)
         const T p0 = v0, p1 = -v1, p2 = -v2, p3 = -v3;
