@@ -505,11 +505,15 @@ constexpr uint32_t kPhiloxMaxAttempts = 1024;
 // Cabibbo-Marinari hit on U with staple A, subgroups (0,1),(1,2),(0,2)
 // (reference src/heatbath/heatbath_mpi.cpp:11-34).  Only the four entries of
 // W = U A in the subgroup block are formed, and only rows i, j of U change.
-// Returns the number of exhausted KP budgets.
+// Sphere uniforms (Philox mode): subgroup m = 3h + g uses words 2(m&1) and
+// 2(m&1)+1 of the call 0x80000000 at the counter of subgroup m - (m&1), so
+// two subgroups share one call; `stash` carries the odd member's two words
+// from one subgroup (or hit) to the next.  Returns the number of exhausted
+// KP budgets.
 template <class T, int MODE>
 __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, int h, uint32_t cb, uint32_t sweep,
                                       uint32_t key0, uint32_t key1, const double* __restrict__ inj, int n_hits, int K,
-                                      uint32_t Vh) {
+                                      uint32_t Vh, uint32_t stash[2]) {
     int exhausted = 0;
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
@@ -547,6 +551,19 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
         bool ok = false;
         const double* u = nullptr;
         const uint32_t cz = hb_ctr_z(hs, h, g);
+        uint32_t sw0 = 0, sw1 = 0;
+        if (MODE != kHbInjected) {
+            if (((3 * h + g) & 1) == 0) {
+                U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
+                sw0 = os.x;
+                sw1 = os.y;
+                stash[0] = os.z;
+                stash[1] = os.w;
+            } else {
+                sw0 = stash[0];
+                sw1 = stash[1];
+            }
+        }
         if (MODE == kHbInjected) {
             const int nslot = 4 * K + 2;
             u = inj + ((size_t)((hs * n_hits + h) * 3 + g) * (size_t)nslot) * Vh + cb;
@@ -575,8 +592,7 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
         if (MODE == kHbInjected) {
             kp_sphere<T>(a0, (T)u[(size_t)(4 * K) * Vh], (T)u[(size_t)(4 * K + 1) * Vh], x);
         } else {
-            U4 os = philox4x32_10(U4{cb, sweep, cz, 0x80000000u}, key0, key1);
-            kp_sphere<T>(a0, KpMath::uni<T>(os.x), KpMath::uni<T>(os.y), x);
+            kp_sphere<T>(a0, KpMath::uni<T>(sw0), KpMath::uni<T>(sw1), x);
         }
         // r = mult(x, adj(v)) (reference src/heatbath/heatbath.cpp:39-44, src/su3/utils.cpp:53-63)
         const T p0 = v0, p1 = -v1, p2 = -v2, p3 = -v3;
@@ -646,8 +662,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         const T beta = (T)L.beta;
         const uint32_t sw = sweep + L.sweep_off;
         int ex = 0;
+        uint32_t stash[2] = {0u, 0u};
         for (int h = 0; h < n_hits; ++h)
-            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh);
+            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh, stash);
         st_link(L.U, L.S, MU, p, i, U);
         if (ex) atomicAdd(exhausted, (unsigned long long)ex);
     }
@@ -760,7 +777,8 @@ __global__ void k_math_probe(int op, int n, const double* __restrict__ in, doubl
         M3<T> U, A;
         for (int k = 0; k < 18; ++k) { U.v[k] = (T)src[k]; A.v[k] = (T)src[18 + k]; }
         // uniforms laid out [g][slot] with stride 1: reuse the injected path with Vh = 1
-        cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1);
+        uint32_t stash[2] = {0u, 0u};
+        cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1, stash);
         for (int k = 0; k < 18; ++k) out[(size_t)t * 18 + k] = (double)U.v[k];
     } else if (op == 5) {
         out[t] = (double)KpMath::uni<T>((uint32_t)in[t]);

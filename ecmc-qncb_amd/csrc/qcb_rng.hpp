@@ -5,9 +5,11 @@
 //   key  = { seed & 0xffffffff, seed >> 32 }
 //   heatbath:  ctr = { cb_site, sweep_idx, (hs<<24)|(hit<<8)|(subgroup<<4), call }
 //              hs = 2*mu + parity; call a feeds Kennedy-Pendleton attempt a with
-//              (delta, r1, r2, r3) = u32(w0..w3); call 0x80000000 feeds the sphere
-//              draw with u32(w0), u32(w1).  u32(w) = (w + 1/2) 2^-32 is exact in
-//              double; the float path rounds that double, so FP32 and FP64 lattices
+//              (delta, r1, r2, r3) = u32(w0..w3).  Sphere draws: with m = 3*hit +
+//              subgroup, the call 0x80000000 at the counter of subgroup m - (m&1)
+//              feeds subgroup m with u32(w[2(m&1)]), u32(w[2(m&1)+1]) -- one call per
+//              two subgroups.  u32(w) = (w + 1/2) 2^-32 is exact in double; the
+//              float path takes the float nearest to it, so FP32 and FP64 lattices
 //              consume identical uniforms.
 //   hot start: ctr = { lex_site, mu, 1, call 0..8 } (nine Box-Muller pairs) with
 //              u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) + 2^-53, strictly in (0,1).

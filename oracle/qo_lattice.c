@@ -558,9 +558,13 @@ static int kp_phx_cb(void* vctx, int g, double beta, double k, double out[4]) {
         if (qo_kp_try(alpha, qo_u32(o[0]), qo_u32(o[1]), qo_u32(o[2]), qo_u32(o[3]), &a0)) break;
     }
     if (a == 1024) return -1;
-    uint32_t cs[4] = {c->cb, c->sweep, c2, 0x80000000u}, os[4];
+    /* sphere: subgroup m = 3h + g reads words 2(m&1), 2(m&1)+1 of the call
+     * 0x80000000 at the counter of subgroup m - (m&1) (one call per two subgroups) */
+    int m = 3 * c->h + g, m0 = m & ~1, wsel = 2 * (m & 1);
+    uint32_t cs2 = (c->hs << 24) | ((uint32_t)(m0 / 3) << 8) | ((uint32_t)(m0 % 3) << 4);
+    uint32_t cs[4] = {c->cb, c->sweep, cs2, 0x80000000u}, os[4];
     qo_philox4x32_10(cs, c->key, os);
-    qo_kp_sphere(a0, qo_u32(os[0]), qo_u32(os[1]), out);
+    qo_kp_sphere(a0, qo_u32(os[wsel]), qo_u32(os[wsel + 1]), out);
     return 0;
 }
 

@@ -14,10 +14,12 @@
  *   key  = { seed & 0xffffffff, seed >> 32 }
  *   heatbath: ctr = { cb_site, sweep_idx, (hs<<24)|(hit<<8)|(subgroup<<4)|0, call }
  *             call a -> attempt a: (delta, r1, r2, r3) = u32(w0..w3);
- *             call 0x80000000 -> sphere: (phi, cos theta) = u32(w0), u32(w1)
+ *             sphere of subgroup m = 3*hit + subgroup: call 0x80000000 at the
+ *             counter of subgroup m - (m&1), (phi, cos theta) = u32(w[2(m&1)]),
+ *             u32(w[2(m&1)+1]) -- two subgroups share one call;
  *             u32(w) = (w + 1/2) 2^-32, exact in double and strictly inside (0,1);
- *             the float path rounds the same double, so both precisions see
- *             the same uniforms.
+ *             the float path takes the float nearest to the same value, so both
+ *             precisions see the same uniforms.
  *   hot start: ctr = { lex_site, mu, 1, call 0..8 }  (Box-Muller pairs)
  *              with u53(w0,w1), u53(w2,w3): top 52 bits of (hi:lo) plus half
  *              an ulp, strictly inside (0,1).
