@@ -153,12 +153,6 @@ extern "C" int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt, 
     return upload_desc(lat);
 }
 
-int qcb_lattice_dims_internal(qcb_lattice* lat, int dims[4]) {
-    if (!lat) return QCB_EINVAL;
-    for (int d = 0; d < 4; ++d) dims[d] = lat->L[d];
-    return QCB_OK;
-}
-
 extern "C" int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes) {
     if (!lat || !bytes) return QCB_EINVAL;
     uint64_t f = field_bytes(lat);

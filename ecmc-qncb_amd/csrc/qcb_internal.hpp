@@ -25,6 +25,9 @@ enum HbMode { kHbPhilox = 0, kHbInjected = 1 };
 template <class T> void launch_cold(const Launch& g);
 template <class T> void launch_hot(const Launch& g);   // key from descriptor
 template <class T> void launch_aos_to_soa(const Launch& g, const double* aos, T* dst_field);
+// ILDG staging of links [lo, lo + n) in file order (single-lattice launches)
+template <class T> void launch_soa_to_be(const Launch& g, const T* src, uint64_t* out, uint64_t lo, uint32_t n);
+template <class T> void launch_be_to_soa(const Launch& g, const uint64_t* in, T* dst, uint64_t lo, uint32_t n);
 template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
 template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
 template <class T> void launch_flow_init(const Launch& g);   // F[0] <- U for every lattice

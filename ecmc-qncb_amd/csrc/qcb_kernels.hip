@@ -181,6 +181,48 @@ __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const LatDev<T>* __restri
     for (int k = 0; k < 18; ++k) d[k] = (double)m.v[k];
 }
 
+// ILDG staging: links [lo, lo + n) in file order (lex site x fastest, then
+// mu; lex = 2 cb + x-parity bit, so cb = lex / 2) as big-endian doubles, one
+// thread per link (reference src/io/ildg.cpp:87-106).  The 18 doubles of a
+// link are 144 contiguous bytes; byte order is swapped on the device so the
+// host only streams the buffer to the file.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_soa_to_be(const LatDev<T>* __restrict__ lats, const T* __restrict__ src,
+                                                      uint64_t* __restrict__ out, uint64_t lo, uint32_t n) {
+    const LatDev<T>& L = lats[0];
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t link = lo + t;
+    const uint32_t lex = (uint32_t)(link >> 2);
+    const int mu = (int)(link & 3u);
+    const Coords c = cb_coords(L, lex >> 1, 0);
+    const int x = (int)(lex % (uint32_t)L.L[0]);
+    const int p = (x + c.y + c.z + c.t) & 1;
+    M3<T> m = ld_link(src, L.S, mu, p, lex >> 1);
+    uint64_t* d = out + (size_t)t * 18;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) d[k] = __builtin_bswap64(__double_as_longlong((double)m.v[k]));
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_be_to_soa(const LatDev<T>* __restrict__ lats, const uint64_t* __restrict__ in,
+                                                      T* __restrict__ dst, uint64_t lo, uint32_t n) {
+    const LatDev<T>& L = lats[0];
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t link = lo + t;
+    const uint32_t lex = (uint32_t)(link >> 2);
+    const int mu = (int)(link & 3u);
+    const Coords c = cb_coords(L, lex >> 1, 0);
+    const int x = (int)(lex % (uint32_t)L.L[0]);
+    const int p = (x + c.y + c.z + c.t) & 1;
+    M3<T> m;
+    const uint64_t* s = in + (size_t)t * 18;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) m.v[k] = (T)__longlong_as_double(__builtin_bswap64(s[k]));
+    st_link(dst, L.S, mu, p, lex >> 1, m);
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, const T* __restrict__ src,
@@ -808,6 +850,16 @@ void launch_soa_to_aos(const Launch& g, const T* src, double* aos) {
     hipLaunchKernelGGL(k_soa_to_aos<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, aos);
 }
 template <class T>
+void launch_soa_to_be(const Launch& g, const T* src, uint64_t* out, uint64_t lo, uint32_t n) {
+    hipLaunchKernelGGL(k_soa_to_be<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, g.stream,
+                       static_cast<const LatDev<T>*>(g.lats), src, out, lo, n);
+}
+template <class T>
+void launch_be_to_soa(const Launch& g, const uint64_t* in, T* dst, uint64_t lo, uint32_t n) {
+    hipLaunchKernelGGL(k_be_to_soa<T>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, g.stream,
+                       static_cast<const LatDev<T>*>(g.lats), in, dst, lo, n);
+}
+template <class T>
 void launch_copy_field(const Launch& g, const T* src, T* dst) {
     hipLaunchKernelGGL(k_copy_field<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, dst);
 }
@@ -896,6 +948,8 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_aos_to_soa<T>(const Launch&, const double*, T*);                                     \
     template void launch_soa_to_aos<T>(const Launch&, const T*, double*);                                     \
     template void launch_copy_field<T>(const Launch&, const T*, T*);                                          \
+    template void launch_soa_to_be<T>(const Launch&, const T*, uint64_t*, uint64_t, uint32_t);                 \
+    template void launch_be_to_soa<T>(const Launch&, const uint64_t*, T*, uint64_t, uint32_t);                 \
     template void launch_flow_init<T>(const Launch&);                                                         \
     template void launch_plaquette<T>(const Launch&, double*);                                                \
     template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
