@@ -70,6 +70,8 @@ void qo_lat_free(qo_lat* l);
 void qo_lat_cold(qo_lat* l);                                            /* GaugeField.cpp:23-33 */
 void qo_lat_hot_ref(qo_lat* l, qo_mt64* g);                             /* GaugeField.cpp:10-20 + utils.cpp:13-30; Householder conventions parity unpinned */
 void qo_lat_hot_philox(qo_lat* l, uint64_t seed);                       /* device hot start restated: Box-Muller + Gram-Schmidt + det^(1/3) */
+void qo_haar_su3_philox(const uint32_t key[2], uint32_t c0, uint32_t c1, uint32_t c2, double* Q); /* the same on counters {c0,c1,c2,0..8} */
+void qo_random_su3_ref(qo_mt64* g, double* Q);                          /* utils.cpp:13-30 (one matrix); Householder conventions parity unpinned */
 void qo_staple(const qo_lat* l, size_t site, int mu, double* out);      /* GaugeField.cpp:74-97 */
 double qo_plaquette(const qo_lat* l);                                   /* observables_mpi.cpp:8-43 */
 void qo_clover_F(const qo_lat* l, size_t site, int mu, int nu, double* F); /* observables_mpi.cpp:61-85 */
@@ -116,6 +118,45 @@ typedef struct {
  * tqe_out: 3*N_steps_gf per flow; shifts_out: 4 ints per shift (therm + sampling). */
 int qo_ref_run(const qo_run_params* p, double* plaq_out, double* tqe_out, int* shifts_out,
                double* final_links, qo_run_stats* st);
+
+/* ---------------- event-chain Monte Carlo (src/ecmc/ecmc_mpi_cb.cpp) ---------------- */
+typedef struct {
+    double beta;
+    double theta_sample, theta_refresh_site, theta_refresh_R;   /* param_theta_* (params.h:9-17) */
+    int poisson;
+    double epsilon_set;
+    int block[4];            /* chain sub-block extents (the reference's L_core per rank) */
+} qo_ecmc_params;
+typedef struct {             /* LocalChainState (ecmc_mpi_cb.h:15-35) plus the Philox position */
+    double R[18];
+    double th_refresh_site, th_refresh_R, par_refresh_site, par_refresh_R;
+    uint64_t events, lifts, revs, set_counter, set_gen, rng_n;
+    size_t site;             /* global lex site */
+    int mu, eps, initialized;
+} qo_ecmc_chain;
+double qo_ecmc_reject_fast(double A, double B, double gamma, int epsilon);   /* ecmc_mpi_cb.h:56-94 */
+double qo_ecmc_reject_exact(double A, double B, double gamma, int epsilon);  /* ecmc_mpi_cb.cpp:82-178 */
+double qo_ecmc_compute_ds(const double* Pi, const double* R);                /* ecmc_mpi_cb.cpp:263-275 */
+void qo_ecmc_update(double* U, double theta, int epsilon, const double* R);  /* ecmc_mpi_cb.cpp:357-372 */
+void qo_ecmc_random_su3_eps(double eps, const double r[9], double* M);       /* utils.cpp:66-103 */
+int qo_ecmc_n_chains(const qo_lat* l, const qo_ecmc_params* p);
+/* One sample (sample_persistant, ecmc_mpi_cb.cpp:496-637) of every chain of
+ * the block decomposition with the device's Philox counters:
+ *   key = seed; ctr(n, call) = {chain, n_lo, 0xEC000000 | n_hi(24 bits), call}
+ *   sample start: n += 1, call 6 -> theta_sample (u53 of w0,w1, Poisson)
+ *   event: n += 1; call 0 -> gamma_0..3, call 1 -> gamma_4,5, select, accept;
+ *          call 2 -> set index (64-bit w0:w1 times 10001, high word);
+ *          refresh site: call 3 -> 4 coordinates 1 + hi32(w * (b-2)), call 4 ->
+ *          dir = w0>>30, eps = 2(w1>>31)-1, theta_refresh_site (w2,w3);
+ *          refresh R: call 5 -> theta_refresh_R (w0,w1), Haar SU(3) on
+ *          {chain, n_lo, 0xED000000 | n_hi, 0..8}
+ *   init: the same draws at n = 0; gamma u = u32(w), exponentials -mean log(u53)
+ *   matrix set, generation g (+1 per sample call and per 10000 lifts):
+ *          element 2m+1 from {chain, g, 0xE5000000 | m, 0..2} (9 words u32 - 1/2),
+ *          element 2m+2 its adjoint, element 0 the identity. */
+int qo_ecmc_sample_philox(qo_lat* l, const qo_ecmc_params* p, qo_ecmc_chain* chains, uint64_t seed);
+/* The reference's one-rank chain (the whole lattice one block), mt19937_64. */
+int qo_ecmc_sample_ref(qo_lat* l, const qo_ecmc_params* p, qo_ecmc_chain* chain, qo_mt64* g);
 
 double qo_now(void);
 

@@ -70,7 +70,7 @@ void qo_lat_cold(qo_lat* l) {
  * (src/su3/utils.cpp:13-30).  Draw order re-then-im per entry is a choice:
  * the reference's Complex(gauss(rng), gauss(rng)) leaves it to the compiler.
  * Parity unpinned (Eigen absent). */
-static void random_su3_ref(qo_mt64* g, double* Q) {
+void qo_random_su3_ref(qo_mt64* g, double* Q) {
     qo_c A[3][3];
     /* a fresh normal_distribution per call: 18 draws are 9 whole polar pairs */
     double gs[18];
@@ -165,7 +165,7 @@ static void random_su3_ref(qo_mt64* g, double* Q) {
 void qo_lat_hot_ref(qo_lat* l, qo_mt64* g) {
     /* interior sites in t,z,y,x order, mu inner (src/gauge/GaugeField.cpp:10-20) */
     for (size_t s = 0; s < l->V; ++s)
-        for (int mu = 0; mu < 4; ++mu) random_su3_ref(g, LNK(l, s, mu));
+        for (int mu = 0; mu < 4; ++mu) qo_random_su3_ref(g, LNK(l, s, mu));
 }
 
 /* Device hot start restated: 18 Box-Muller Gaussians from Philox
@@ -173,9 +173,14 @@ void qo_lat_hot_ref(qo_lat* l, qo_mt64* g) {
  * (Haar on U(3)), then multiplication by det^(-1/3) (principal branch). */
 void qo_hot_link_philox(uint64_t seed, uint32_t site, uint32_t mu, double* Q) {
     uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    qo_haar_su3_philox(key, site, mu, 1u, Q);
+}
+
+/* The same generator on counters {c0, c1, c2, call 0..8}. */
+void qo_haar_su3_philox(const uint32_t key[2], uint32_t c0, uint32_t c1, uint32_t c2, double* Q) {
     double gz[18];
     for (uint32_t c = 0; c < 9; ++c) {
-        uint32_t ctr[4] = {site, mu, 1u, c}, o[4];
+        uint32_t ctr[4] = {c0, c1, c2, c}, o[4];
         qo_philox4x32_10(ctr, key, o);
         double u1 = qo_u53(o[0], o[1]), u2 = qo_u53(o[2], o[3]);
         double r = sqrt(-2.0 * log(u1));

@@ -34,6 +34,25 @@ class RunParams(C.Structure):
             "N_rk_steps")]
 
 
+class EcmcParams(C.Structure):
+    _fields_ = [("beta", C.c_double), ("theta_sample", C.c_double), ("theta_refresh_site", C.c_double),
+                ("theta_refresh_R", C.c_double), ("poisson", C.c_int), ("epsilon_set", C.c_double),
+                ("block", C.c_int * 4)]
+
+
+class EcmcChain(C.Structure):
+    _fields_ = [("R", C.c_double * 18), ("th_refresh_site", C.c_double), ("th_refresh_R", C.c_double),
+                ("par_refresh_site", C.c_double), ("par_refresh_R", C.c_double)] + [
+        (n, C.c_uint64) for n in ("events", "lifts", "revs", "set_counter", "set_gen", "rng_n")] + [
+        ("site", C.c_size_t), ("mu", C.c_int), ("eps", C.c_int), ("initialized", C.c_int)]
+
+
+def ecmc_params(beta=6.0, theta_sample=100.0, theta_refresh_site=50.0, theta_refresh_R=15.0, poisson=False,
+                epsilon_set=0.15, block=(4, 4, 4, 4)):
+    return EcmcParams(beta, theta_sample, theta_refresh_site, theta_refresh_R, int(poisson), epsilon_set,
+                      (C.c_int * 4)(*block))
+
+
 class RunStats(C.Structure):
     _fields_ = [("t_hb", C.c_double), ("t_plaq", C.c_double), ("t_flow", C.c_double),
                 ("t_shift", C.c_double), ("link_visits", C.c_long), ("site_steps", C.c_long),
@@ -91,6 +110,15 @@ def lib():
             "qo_hb_sweep_cb_philox": (None, [P(QoLat), C.c_double, C.c_int, C.c_uint64, C.c_uint32]),
             "qo_ref_run": (C.c_int, [P(RunParams), _D, _D, P(C.c_int), _D, P(RunStats)]),
             "qo_now": (C.c_double, []),
+            "qo_ecmc_reject_fast": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
+            "qo_ecmc_reject_exact": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
+            "qo_ecmc_compute_ds": (C.c_double, [_D, _D]),
+            "qo_ecmc_update": (None, [_D, C.c_double, C.c_int, _D]),
+            "qo_ecmc_random_su3_eps": (None, [C.c_double, _D, _D]),
+            "qo_ecmc_n_chains": (C.c_int, [P(QoLat), P(EcmcParams)]),
+            "qo_ecmc_sample_philox": (C.c_int, [P(QoLat), P(EcmcParams), P(EcmcChain), C.c_uint64]),
+            "qo_ecmc_sample_ref": (C.c_int, [P(QoLat), P(EcmcParams), P(EcmcChain), P(QoMt64)]),
+            "qo_haar_su3_philox": (None, [P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, _D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -203,6 +231,17 @@ class Lattice:
 
     def sweep_philox(self, beta, n_hits, seed, sweep_idx):
         lib().qo_hb_sweep_cb_philox(self.ref(), beta, n_hits, seed, sweep_idx)
+
+    def ecmc_philox(self, params: EcmcParams, chains, seed: int) -> int:
+        return lib().qo_ecmc_sample_philox(self.ref(), C.byref(params), chains, seed)
+
+    def ecmc_ref(self, params: EcmcParams, chain: EcmcChain, mt: "Mt64") -> int:
+        return lib().qo_ecmc_sample_ref(self.ref(), C.byref(params), C.byref(chain), C.byref(mt.s))
+
+    def ecmc_chains(self, params: EcmcParams):
+        n = lib().qo_ecmc_n_chains(self.ref(), C.byref(params))
+        assert n > 0
+        return (EcmcChain * n)()
 
     def copy(self) -> "Lattice":
         o = Lattice(self.dims)
