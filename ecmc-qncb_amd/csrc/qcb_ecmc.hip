@@ -1,0 +1,553 @@
+// qcb_ecmc.hip -- event-chain Monte Carlo on the GPU (reference
+// src/ecmc/ecmc_mpi_cb.cpp:12-637, src/ecmc/ecmc_mpi_cb.h:56-94).
+//
+// The reference runs one event chain per MPI rank on its L_core^4 block, with
+// the block's "c = 0" faces frozen (src/geometry/GeometryCB.cpp:33-56) and
+// random shifts between samples.  Here the whole lattice sits on one device
+// and is cut into the same blocks: a link (x, mu) is frozen when
+// x_d mod b_d == 0 for some d != mu, a non-frozen link of a block only reads
+// links of its own block or frozen ones, so every block's chain runs at once,
+// one chain per thread.  Chains keep their LocalChainState between samples
+// (sample_persistant) in device memory.  Randomness is Philox4x32-10 on
+// per-chain counters (scheme in include/qcb.h), so each chain's trajectory
+// depends only on (seed, chain index, its own history).  FP64 links.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/qcb.h"
+#include "qcb_host.hpp"
+#include "qcb_layout.hpp"
+#include "qcb_rng.hpp"
+#include "qcb_su3.hpp"
+
+using namespace qcb;
+
+namespace {
+
+// LocalChainState (ecmc_mpi_cb.h:15-35) plus the chain's Philox position.
+struct EcmcChainDev {
+    double R[18];
+    double th_rs, th_rR, par_rs, par_rR;
+    unsigned long long events, lifts, revs, set_counter, set_gen, rng_n;
+    uint32_t site;
+    int mu, eps, initialized, status, pad;
+};
+
+struct EcmcArgs {
+    double beta, theta_sample, theta_refresh_site, theta_refresh_R, epsilon_set;
+    int poisson;
+    int b[4], nb[4], L[4];
+    uint32_t key0, key1;
+    unsigned long long max_events;
+    int n_chains;
+};
+
+constexpr uint64_t kSetSize = 10001;   // ecmc_set (src/su3/utils.cpp:119-127), 1 + 5000 pairs
+constexpr int kChainsPerBlock = 64;
+
+__device__ inline int dsgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
+
+__device__ inline void gco(const EcmcArgs& a, uint32_t s, int c[4]) {
+    c[0] = (int)(s % (uint32_t)a.L[0]);
+    s /= (uint32_t)a.L[0];
+    c[1] = (int)(s % (uint32_t)a.L[1]);
+    s /= (uint32_t)a.L[1];
+    c[2] = (int)(s % (uint32_t)a.L[2]);
+    c[3] = (int)(s / (uint32_t)a.L[2]);
+}
+
+__device__ inline uint32_t glx(const EcmcArgs& a, const int c[4]) {
+    return (uint32_t)c[0] + (uint32_t)a.L[0] * ((uint32_t)c[1] + (uint32_t)a.L[1] * ((uint32_t)c[2] + (uint32_t)a.L[2] * (uint32_t)c[3]));
+}
+
+__device__ inline uint32_t gsh(const EcmcArgs& a, uint32_t s, int d1, int s1, int d2 = 0, int s2 = 0) {
+    int c[4];
+    gco(a, s, c);
+    if (s1) c[d1] = (c[d1] + s1 + a.L[d1]) % a.L[d1];
+    if (s2) c[d2] = (c[d2] + s2 + a.L[d2]) % a.L[d2];
+    return glx(a, c);
+}
+
+__device__ inline bool frozen(const EcmcArgs& a, uint32_t s, int mu) {
+    int c[4];
+    gco(a, s, c);
+    bool f = false;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) f = f || (d != mu && c[d] % a.b[d] == 0);
+    return f;
+}
+
+// links by lex site: cb index = lex / 2 (Lx even), parity = coordinate sum
+__device__ inline M3<double> ldl(const LatDev<double>& L, const EcmcArgs& a, uint32_t s, int mu) {
+    int c[4];
+    gco(a, s, c);
+    return ld_link(L.U, L.S, mu, (c[0] + c[1] + c[2] + c[3]) & 1, s >> 1);
+}
+
+__device__ inline void stl(const LatDev<double>& L, const EcmcArgs& a, uint32_t s, int mu, const M3<double>& m) {
+    int c[4];
+    gco(a, s, c);
+    double2* b = (double2*)L.U + link_base(L.S, mu * 2 + ((c[0] + c[1] + c[2] + c[3]) & 1), s >> 1);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) b[64 * k] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
+}
+
+__device__ inline M3<double> dag(const M3<double>& a) {
+    M3<double> o;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            o.re(i, j) = a.re(j, i);
+            o.im(i, j) = -a.im(j, i);
+        }
+    return o;
+}
+
+// solve_reject_fast (ecmc_mpi_cb.h:56-94)
+__device__ double reject_fast(double A, double B, double gamma, int eps) {
+    B = (eps == -1) ? -B : B;
+    const double R = hypot(A, B);
+    const double invR = 1.0 / R;
+    const double period = 2.0 * R;
+    const double discarded = floor(gamma / period);
+    gamma -= discarded * period;
+    double phi = atan2(-A, B);
+    phi += (phi < 0.0) ? (2.0 * M_PI) : 0.0;
+    const double p1 = R - A;
+    double alpha;
+    if (phi < (M_PI * 0.5) || phi > (M_PI * 1.5))
+        alpha = (gamma > p1) ? (gamma - p1) * invR - 1.0 : (gamma + A) * invR;
+    else
+        alpha = gamma * invR - 1.0;
+    alpha = (alpha > 1.0) ? 1.0 : ((alpha < -1.0) ? -1.0 : alpha);
+    double theta = phi + asin(alpha);
+    theta += (theta < 0.0) ? (2.0 * M_PI) : 0.0;
+    theta -= (theta >= 2.0 * M_PI) ? (2.0 * M_PI) : 0.0;
+    return theta + 2.0 * M_PI * discarded;
+}
+
+// compute_ds (ecmc_mpi_cb.cpp:263-275): Im[(R^dag P R)_00 - (R^dag P R)_11];
+// only columns 0 and 1 of P R are formed
+__device__ double compute_ds(const M3<double>& P, const M3<double>& R) {
+    double res = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int col = 0; col < 2; ++col) {
+            double mr = 0.0, mi = 0.0;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+                mr += P.re(k, l) * R.re(l, col) - P.im(k, l) * R.im(l, col);
+                mi += P.re(k, l) * R.im(l, col) + P.im(k, l) * R.re(l, col);
+            }
+            const double t = R.re(k, col) * mi - R.im(k, col) * mr;
+            res += col == 0 ? t : -t;
+        }
+    }
+    return res;
+}
+
+// update (ecmc_mpi_cb.cpp:357-372): U <- R diag(e^{i xi}, e^{-i xi}, 1) R^dag U
+__device__ M3<double> ecmc_update(const M3<double>& U, double theta, int eps, const M3<double>& R) {
+    M3<double> T = mul_dn(R, U);
+    const double xi = eps * theta;
+    double s, c;
+    sincos(xi, &s, &c);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        double a = T.re(0, j), b = T.im(0, j);
+        T.re(0, j) = a * c - b * s;
+        T.im(0, j) = a * s + b * c;
+        a = T.re(1, j);
+        b = T.im(1, j);
+        T.re(1, j) = a * c + b * s;
+        T.im(1, j) = -a * s + b * c;
+    }
+    return mul_nn(R, T);
+}
+
+__device__ inline U4 ctr_ev(uint32_t chain, unsigned long long n, uint32_t call, const EcmcArgs& a) {
+    return philox4x32_10(U4{chain, (uint32_t)n, 0xEC000000u | ((uint32_t)(n >> 32) & 0x00FFFFFFu), call}, a.key0,
+                         a.key1);
+}
+
+// random_SU3_epsilon (src/su3/utils.cpp:66-103): X01 X02 X12 near the identity
+__device__ M3<double> random_su3_eps(double eps, const double r[9]) {
+    M3<double> M;
+    m3_identity(M);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int i = f == 2 ? 1 : 0, j = f == 0 ? 1 : 2, k = 3 - i - j;
+        const double n = sqrt(r[3 * f] * r[3 * f] + r[3 * f + 1] * r[3 * f + 1] + r[3 * f + 2] * r[3 * f + 2]);
+        const double a0 = sqrt(1 - eps * eps), a1 = eps * r[3 * f] / n, a2 = eps * r[3 * f + 1] / n,
+                     a3 = eps * r[3 * f + 2] / n;
+        M3<double> X;
+        m3_zero(X);
+        X.re(k, k) = 1.0;
+        X.re(i, i) = a0; X.im(i, i) = a3;
+        X.re(j, j) = a0; X.im(j, j) = -a3;
+        X.re(i, j) = a2; X.im(i, j) = a1;
+        X.re(j, i) = -a2; X.im(j, i) = a1;
+        M = mul_nn(M, X);
+    }
+    return M;
+}
+
+// element i of the chain's matrix set of generation g: 0 the identity,
+// 2m+1 a near-identity matrix from {chain, g, 0xE5000000 | m, 0..2}, 2m+2 its adjoint
+__device__ M3<double> set_element(uint32_t chain, unsigned long long g, unsigned long long i, const EcmcArgs& a) {
+    M3<double> M;
+    if (i == 0) {
+        m3_identity(M);
+        return M;
+    }
+    const uint32_t m = (uint32_t)((i - 1) / 2);
+    double r[9];
+#pragma unroll
+    for (uint32_t k = 0; k < 3; ++k) {
+        U4 o = philox4x32_10(U4{chain, (uint32_t)g, 0xE5000000u | m, k}, a.key0, a.key1);
+        r[3 * k] = u32d(o.x) - 0.5;
+        r[3 * k + 1] = u32d(o.y) - 0.5;
+        r[3 * k + 2] = u32d(o.z) - 0.5;
+    }
+    M = random_su3_eps(a.epsilon_set, r);
+    return (i & 1) ? M : dag(M);
+}
+
+// staple j of link (x, mu) (ecmc_mpi_cb.cpp:12-36): nu = j/2-th direction != mu
+__device__ inline int dir_of(int mu, int j) {
+    const int k = j >> 1;
+    return k + (k >= mu ? 1 : 0);
+}
+
+__device__ M3<double> staple_j(const LatDev<double>& L, const EcmcArgs& a, uint32_t x, int mu, int j) {
+    const int nu = dir_of(mu, j);
+    if ((j & 1) == 0) {
+        M3<double> t = mul_nn(ldl(L, a, x, nu), ldl(L, a, gsh(a, x, nu, 1), mu));
+        return mul_nd(ldl(L, a, gsh(a, x, mu, 1), nu), t);
+    }
+    const uint32_t xmnu = gsh(a, x, nu, -1);
+    M3<double> t = mul_nn(ldl(L, a, xmnu, mu), ldl(L, a, gsh(a, x, mu, 1, nu, -1), nu));
+    return mul_dn(t, ldl(L, a, xmnu, nu));
+}
+
+// the four links of plaquette j, the current link first (GeometryCB.cpp:60-90)
+__device__ inline void plaq_links(const EcmcArgs& a, uint32_t x, int mu, int j, uint32_t s[4], int d[4]) {
+    const int nu = dir_of(mu, j);
+    s[0] = x;
+    d[0] = mu;
+    if ((j & 1) == 0) {
+        s[1] = gsh(a, x, mu, 1); d[1] = nu;
+        s[2] = gsh(a, x, nu, 1); d[2] = mu;
+        s[3] = x;                d[3] = nu;
+    } else {
+        s[1] = gsh(a, x, mu, 1, nu, -1); d[1] = nu;
+        s[2] = gsh(a, x, nu, -1);        d[2] = mu;
+        s[3] = s[2];                     d[3] = nu;
+    }
+}
+
+// plaquette matrix P_i of lift_improved_fast (ecmc_mpi_cb.cpp:296-318)
+__device__ M3<double> plaq_P(int i, bool fwd, const M3<double>& U0, const M3<double>& U1, const M3<double>& U2,
+                             const M3<double>& U3) {
+    if (fwd) {
+        const M3<double> U01 = mul_nn(U0, U1), U32 = mul_nn(U3, U2);
+        if (i == 0) return mul_nd(U01, U32);
+        if (i == 1) return mul_nn(mul_nd(U1, U32), U0);
+        if (i == 2) return mul_nn(mul_nd(U2, U01), U3);
+        return dag(mul_nd(U01, U32));
+    }
+    const M3<double> T = mul_nd(U0, mul_nn(U2, U1));
+    if (i == 0) return mul_nn(T, U3);
+    if (i == 1) return mul_nn(mul_nd(mul_nd(U1, U0), U3), U2);
+    if (i == 3) return mul_nn(U3, T);
+    return dag(mul_nn(U3, T));
+}
+
+__global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<double>* __restrict__ lat,
+                                                                 EcmcChainDev* __restrict__ chains, EcmcArgs a) {
+    const int cid = blockIdx.x * kChainsPerBlock + threadIdx.x;
+    if (cid >= a.n_chains) return;
+    const LatDev<double>& L = *lat;
+    const uint32_t chain = (uint32_t)cid;
+    EcmcChainDev st = chains[cid];
+    int org[4];
+    {
+        int r = cid;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            org[d] = (r % a.nb[d]) * a.b[d];
+            r /= a.nb[d];
+        }
+    }
+    M3<double> R;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) R.v[k] = st.R[k];
+    auto draw_site = [&](unsigned long long n) {
+        const U4 o3 = ctr_ev(chain, n, 3, a), o4 = ctr_ev(chain, n, 4, a);
+        const uint32_t w[4] = {o3.x, o3.y, o3.z, o3.w};
+        int c[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) c[d] = org[d] + 1 + (int)__umulhi(w[d], (uint32_t)(a.b[d] - 2));
+        st.site = glx(a, c);
+        st.mu = (int)(o4.x >> 30);
+        st.eps = 2 * (int)(o4.y >> 31) - 1;
+        return o4;
+    };
+    if (!st.initialized) {   // sample_persistant's first call (ecmc_mpi_cb.cpp:504-518)
+        const U4 o4 = draw_site(0);
+        const U4 o5 = ctr_ev(chain, 0, 5, a);
+        R = haar_su3_philox(a.key0, a.key1, chain, 0u, 0xED000000u);
+        st.th_rs = a.poisson ? -a.theta_refresh_site * log(u53(o4.z, o4.w)) : a.theta_refresh_site;
+        st.th_rR = a.poisson ? -a.theta_refresh_R * log(u53(o5.x, o5.y)) : a.theta_refresh_R;
+        st.par_rs = st.par_rR = 0.0;
+        st.set_counter = st.events = st.lifts = 0;
+        st.initialized = 1;
+    }
+    unsigned long long n = st.rng_n + 1;   // theta_sample of this call
+    double th_sample;
+    {
+        const U4 o6 = ctr_ev(chain, n, 6, a);
+        th_sample = a.poisson ? -a.theta_sample * log(u53(o6.x, o6.y)) : a.theta_sample;
+    }
+    unsigned long long gen = st.set_gen + 1;   // the set of this call
+    const double beta_red = -(a.beta / 3.0);
+    double par_sample = 0.0;
+    st.status = 0;
+    for (unsigned long long guard = 0;; ++guard) {
+        if (guard >= a.max_events) {
+            st.status = 1;
+            break;
+        }
+        if (st.set_counter % 10000 == 0) {
+            gen += 1;
+            st.set_counter = 0;
+        }
+        n += 1;
+        const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
+        const double gam[6] = {-log(u32d(o0.x)), -log(u32d(o0.y)), -log(u32d(o0.z)),
+                               -log(u32d(o0.w)), -log(u32d(o1.x)), -log(u32d(o1.y))};
+        // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
+        const M3<double> Ucur = ldl(L, a, st.site, st.mu);
+        const M3<double> T = mul_dn(R, Ucur);
+        int jmin = 0;
+        double theta_reject = 0.0;
+        for (int j = 0; j < 6; ++j) {
+            const M3<double> S = staple_j(L, a, st.site, st.mu, j);
+            double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
+#pragma unroll
+            for (int col = 0; col < 3; ++col) {
+                double m0r = 0, m0i = 0, m1r = 0, m1i = 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    m0r += T.re(0, k) * S.re(k, col) - T.im(0, k) * S.im(k, col);
+                    m0i += T.re(0, k) * S.im(k, col) + T.im(0, k) * S.re(k, col);
+                    m1r += T.re(1, k) * S.re(k, col) - T.im(1, k) * S.im(k, col);
+                    m1i += T.re(1, k) * S.im(k, col) + T.im(1, k) * S.re(k, col);
+                }
+                p00r += m0r * R.re(col, 0) - m0i * R.im(col, 0);
+                p00i += m0r * R.im(col, 0) + m0i * R.re(col, 0);
+                p11r += m1r * R.re(col, 1) - m1i * R.im(col, 1);
+                p11i += m1r * R.im(col, 1) + m1i * R.re(col, 1);
+            }
+            const double rej = reject_fast((p00r + p11r) * beta_red, (p11i - p00i) * beta_red, gam[j], st.eps);
+            if (j == 0 || rej < theta_reject) {
+                theta_reject = rej;
+                jmin = j;
+            }
+        }
+        if (!(theta_reject >= 0.0)) {   // a non-finite angle would never reach the sample event
+            st.status = 2;
+            break;
+        }
+        const double d_sample = th_sample - par_sample;
+        const double d_rs = st.th_rs - st.par_rs;
+        const double d_rR = st.th_rR - st.par_rR;
+        double step = theta_reject;
+        step = d_sample < step ? d_sample : step;
+        step = d_rs < step ? d_rs : step;
+        step = d_rR < step ? d_rR : step;
+        if (step == d_sample) {   // SAMPLE: the chain stops here until the next call
+            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_sample, st.eps, R));
+            st.events++;
+            st.par_rs += d_sample;
+            st.par_rR += d_sample;
+            break;
+        } else if (step == d_rs) {   // REFRESH SITE
+            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_rs, st.eps, R));
+            st.events++;
+            par_sample += d_rs;
+            st.par_rR += d_rs;
+            st.par_rs = 0.0;
+            const U4 o4 = draw_site(n);
+            if (a.poisson) st.th_rs = -a.theta_refresh_site * log(u53(o4.z, o4.w));
+        } else if (step == d_rR) {   // REFRESH R
+            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_rR, st.eps, R));
+            st.events++;
+            par_sample += d_rR;
+            st.par_rs += d_rR;
+            st.par_rR = 0.0;
+            const U4 o5 = ctr_ev(chain, n, 5, a);
+            if (a.poisson) st.th_rR = -a.theta_refresh_R * log(u53(o5.x, o5.y));
+            R = haar_su3_philox(a.key0, a.key1, chain, (uint32_t)n, 0xED000000u | ((uint32_t)(n >> 32) & 0x00FFFFFFu));
+        } else {   // LIFT (lift_improved_fast, ecmc_mpi_cb.cpp:277-354)
+            const M3<double> U0 = ecmc_update(Ucur, theta_reject, st.eps, R);
+            stl(L, a, st.site, st.mu, U0);
+            st.events++;
+            par_sample += theta_reject;
+            st.par_rs += theta_reject;
+            st.par_rR += theta_reject;
+            uint32_t ls[4];
+            int ld[4];
+            plaq_links(a, st.site, st.mu, jmin, ls, ld);
+            const M3<double> U1 = ldl(L, a, ls[1], ld[1]), U2 = ldl(L, a, ls[2], ld[2]), U3 = ldl(L, a, ls[3], ld[3]);
+            const bool fwd = (jmin & 1) == 0;
+            double absd[4], sum = 0.0;
+            int sgn[4];
+            for (int i = 0; i < 4; ++i) {
+                const double ds = frozen(a, ls[i], ld[i]) ? 0.0 : compute_ds(plaq_P(i, fwd, U0, U1, U2, U3), R);
+                sgn[i] = dsgn(ds);
+                absd[i] = fabs(ds);
+                sum += absd[i];
+            }
+            double prob[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) prob[i] = absd[i] / sum;
+            const double u = u32d(o1.z);   // selectVariable (ecmc_mpi_cb.cpp:252-260)
+            const int idx = u < prob[0] ? 0 : (u < prob[0] + prob[1] ? 1 : (u < prob[0] + prob[1] + prob[2] ? 2 : 3));
+            const U4 o2 = ctr_ev(chain, n, 2, a);
+            const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
+            const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
+            const M3<double> Pidx = plaq_P(idx, fwd, U0, U1, U2, U3);
+            const double dS = sgn[idx] * absd[idx];
+            const double dSn = compute_ds(Pidx, Rn);
+            double neps = -sgn[idx];
+            if (fabs(dS) < fabs(dSn) || u32d(o1.w) < fabs(dSn) / fabs(dS)) {
+                R = Rn;
+                neps = -dsgn(dSn);
+            }
+            st.revs += (ls[idx] == st.site) ? 1 : 0;
+            st.set_counter++;
+            st.lifts++;
+            st.site = ls[idx];
+            st.mu = ld[idx];
+            st.eps = (int)neps;
+        }
+    }
+    st.rng_n = n;
+    st.set_gen = gen;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) st.R[k] = R.v[k];
+    chains[cid] = st;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ C-ABI -----
+
+struct qcb_ecmc {
+    qcb_lattice* lat = nullptr;
+    EcmcArgs args{};
+    int n_chains = 0;
+    EcmcChainDev* d_chains = nullptr;
+};
+
+extern "C" int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_ecmc** out) {
+    if (!lat || !p || !out) return QCB_EINVAL;
+    *out = nullptr;
+    qcb_ctx* ctx = lat->ctx;
+    if (lat->prec != QCB_F64) return qcb_set_error(ctx, QCB_EINVAL, "event chains run on FP64 lattices");
+    if (!(p->beta > 0) || !(p->theta_sample > 0) || !(p->theta_refresh_site > 0) || !(p->theta_refresh_R > 0) ||
+        !(p->epsilon_set >= 0 && p->epsilon_set <= 1))
+        return qcb_set_error(ctx, QCB_EINVAL, "ECMC parameters out of range");
+    EcmcArgs a{};
+    int n = 1;
+    for (int d = 0; d < 4; ++d) {
+        if (p->block[d] < 4 || lat->L[d] % p->block[d])
+            return qcb_set_error(ctx, QCB_EINVAL, "block extent %d must be >= 4 and divide %d", p->block[d], lat->L[d]);
+        a.b[d] = p->block[d];
+        a.nb[d] = lat->L[d] / p->block[d];
+        a.L[d] = lat->L[d];
+        n *= a.nb[d];
+    }
+    a.beta = p->beta;
+    a.theta_sample = p->theta_sample;
+    a.theta_refresh_site = p->theta_refresh_site;
+    a.theta_refresh_R = p->theta_refresh_R;
+    a.epsilon_set = p->epsilon_set;
+    a.poisson = p->poisson ? 1 : 0;
+    a.key0 = (uint32_t)p->seed;
+    a.key1 = (uint32_t)(p->seed >> 32);
+    a.max_events = p->max_events ? p->max_events : 1000000000ull;
+    a.n_chains = n;
+    qcb_ecmc* e = new qcb_ecmc();
+    e->lat = lat;
+    e->args = a;
+    e->n_chains = n;
+    if (hipMalloc((void**)&e->d_chains, sizeof(EcmcChainDev) * n) != hipSuccess ||
+        hipMemset(e->d_chains, 0, sizeof(EcmcChainDev) * n) != hipSuccess) {
+        if (e->d_chains) (void)hipFree(e->d_chains);
+        delete e;
+        return qcb_set_error(ctx, QCB_ENOMEM, "cannot allocate %d chain states", n);
+    }
+    ctx->n_live++;
+    *out = e;
+    return QCB_OK;
+}
+
+extern "C" int qcb_ecmc_destroy(qcb_ecmc* e) {
+    if (!e) return QCB_EINVAL;
+    qcb_ctx* ctx = e->lat->ctx;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(e->d_chains);
+    ctx->n_live--;
+    delete e;
+    return QCB_OK;
+}
+
+extern "C" int qcb_ecmc_n_chains(qcb_ecmc* e, int* n) {
+    if (!e || !n) return QCB_EINVAL;
+    *n = e->n_chains;
+    return QCB_OK;
+}
+
+extern "C" int qcb_ecmc_sample(qcb_ecmc* e, int n_samples) {
+    if (!e || n_samples < 0) return QCB_EINVAL;
+    qcb_lattice* lat = e->lat;
+    qcb_ctx* ctx = lat->ctx;
+    const int blocks = (e->n_chains + kChainsPerBlock - 1) / kChainsPerBlock;
+    for (int s = 0; s < n_samples; ++s)
+        hipLaunchKernelGGL(k_ecmc_sample, dim3(blocks), dim3(kChainsPerBlock), 0, ctx->stream,
+                           (const LatDev<double>*)lat->d_desc, e->d_chains, e->args);
+    QCB_HIP(ctx, hipGetLastError());
+    return QCB_OK;
+}
+
+extern "C" int qcb_ecmc_get_chains(qcb_ecmc* e, double* out) {
+    if (!e || !out) return QCB_EINVAL;
+    qcb_ctx* ctx = e->lat->ctx;
+    std::vector<EcmcChainDev> h(e->n_chains);
+    QCB_HIP(ctx, hipMemcpyAsync(h.data(), e->d_chains, sizeof(EcmcChainDev) * h.size(), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    int bad = 0;
+    for (int c = 0; c < e->n_chains; ++c) {
+        const EcmcChainDev& s = h[c];
+        double* o = out + (size_t)c * QCB_ECMC_CHAIN_DOUBLES;
+        for (int k = 0; k < 18; ++k) o[k] = s.R[k];
+        o[18] = s.th_rs; o[19] = s.th_rR; o[20] = s.par_rs; o[21] = s.par_rR;
+        o[22] = (double)s.events; o[23] = (double)s.lifts; o[24] = (double)s.revs;
+        o[25] = (double)s.set_counter; o[26] = (double)s.set_gen; o[27] = (double)s.rng_n;
+        o[28] = (double)s.site; o[29] = s.mu; o[30] = s.eps; o[31] = s.initialized;
+        bad += s.status != 0;
+    }
+    if (bad)
+        return qcb_set_error(ctx, QCB_EBUDGET,
+                             "%d chains stopped before their sample event (max_events reached or a non-finite "
+                             "rejection angle)", bad);
+    return QCB_OK;
+}

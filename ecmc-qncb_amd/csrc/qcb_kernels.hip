@@ -68,8 +68,6 @@ __global__ __launch_bounds__(kBlock) void k_cold(const LatDev<T>* __restrict__ l
     st_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i, m);
 }
 
-// Haar SU(3) from Philox: 18 Box-Muller Gaussians, modified Gram-Schmidt on
-// the columns, then det^(-1/3).  All in double, stored as T.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_hot(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                 int n_lat, uint32_t nblocks) {
@@ -81,64 +79,10 @@ __global__ __launch_bounds__(kBlock) void k_hot(const LatDev<T>* __restrict__ la
     int mu = (int)(hs >> 1), p = (int)(hs & 1);
     Coords c = cb_coords(L, i, p);
     uint32_t lex = (uint32_t)c.x + (uint32_t)L.L[0] * ((uint32_t)c.y + (uint32_t)L.L[1] * ((uint32_t)c.z + (uint32_t)L.L[2] * (uint32_t)c.t));
-    double g[18];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        U4 o = philox4x32_10(U4{lex, (uint32_t)mu, 1u, (uint32_t)k}, L.key0, L.key1);
-        double u1 = u53(o.x, o.y), u2 = u53(o.z, o.w);
-        double r = sqrt(-2.0 * log(u1));
-        double sn, cs;
-        sincos(2.0 * M_PI * u2, &sn, &cs);
-        g[2 * k] = r * cs;
-        g[2 * k + 1] = r * sn;
-    }
-    M3<double> q;
-#pragma unroll
-    for (int k = 0; k < 18; ++k) q.v[k] = g[k];
-    // modified Gram-Schmidt over columns 0,1,2
-#pragma unroll
-    for (int col = 0; col < 3; ++col) {
-#pragma unroll
-        for (int pc = 0; pc < col; ++pc) {
-            double dr = 0, di = 0;
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                dr += q.re(r, pc) * q.re(r, col) + q.im(r, pc) * q.im(r, col);
-                di += q.re(r, pc) * q.im(r, col) - q.im(r, pc) * q.re(r, col);
-            }
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                double a = q.re(r, pc) * dr - q.im(r, pc) * di;
-                double b = q.re(r, pc) * di + q.im(r, pc) * dr;
-                q.re(r, col) -= a;
-                q.im(r, col) -= b;
-            }
-        }
-        double n = 0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) n += q.re(r, col) * q.re(r, col) + q.im(r, col) * q.im(r, col);
-        n = 1.0 / sqrt(n);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) { q.re(r, col) *= n; q.im(r, col) *= n; }
-    }
-    // det by cofactors along row 0
-    double c0r = (q.re(1, 1) * q.re(2, 2) - q.im(1, 1) * q.im(2, 2)) - (q.re(1, 2) * q.re(2, 1) - q.im(1, 2) * q.im(2, 1));
-    double c0i = (q.re(1, 1) * q.im(2, 2) + q.im(1, 1) * q.re(2, 2)) - (q.re(1, 2) * q.im(2, 1) + q.im(1, 2) * q.re(2, 1));
-    double c1r = (q.re(1, 2) * q.re(2, 0) - q.im(1, 2) * q.im(2, 0)) - (q.re(1, 0) * q.re(2, 2) - q.im(1, 0) * q.im(2, 2));
-    double c1i = (q.re(1, 2) * q.im(2, 0) + q.im(1, 2) * q.re(2, 0)) - (q.re(1, 0) * q.im(2, 2) + q.im(1, 0) * q.re(2, 2));
-    double c2r = (q.re(1, 0) * q.re(2, 1) - q.im(1, 0) * q.im(2, 1)) - (q.re(1, 1) * q.re(2, 0) - q.im(1, 1) * q.im(2, 0));
-    double c2i = (q.re(1, 0) * q.im(2, 1) + q.im(1, 0) * q.re(2, 1)) - (q.re(1, 1) * q.im(2, 0) + q.im(1, 1) * q.re(2, 0));
-    double dre = q.re(0, 0) * c0r - q.im(0, 0) * c0i + q.re(0, 1) * c1r - q.im(0, 1) * c1i + q.re(0, 2) * c2r - q.im(0, 2) * c2i;
-    double dim = q.re(0, 0) * c0i + q.im(0, 0) * c0r + q.re(0, 1) * c1i + q.im(0, 1) * c1r + q.re(0, 2) * c2i + q.im(0, 2) * c2r;
-    double ph = -atan2(dim, dre) / 3.0;
-    double sn, cs;
-    sincos(ph, &sn, &cs);
+    const M3<double> q = haar_su3_philox(L.key0, L.key1, lex, (uint32_t)mu, 1u);
     M3<T> out;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        out.v[2 * k] = (T)(q.v[2 * k] * cs - q.v[2 * k + 1] * sn);
-        out.v[2 * k + 1] = (T)(q.v[2 * k] * sn + q.v[2 * k + 1] * cs);
-    }
+    for (int k = 0; k < 18; ++k) out.v[k] = (T)q.v[k];
     st_link(L.U, L.S, mu, p, i, out);
 }
 

@@ -7,6 +7,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "qcb_rng.hpp"
+
 namespace qcb {
 
 template <class T>
@@ -423,6 +425,70 @@ __device__ __forceinline__ void reunitarize_d(M3<double>& u) {
         u.re(i, 2) = re;
         u.im(i, 2) = -im;
     }
+}
+
+// Haar SU(3) from Philox4x32-10 on counters {c0, c1, c2, call 0..8}: 18
+// Box-Muller Gaussians (u53 pairs), modified Gram-Schmidt on the columns
+// (Haar on U(3)), then multiplication by det^(-1/3) on the principal branch.
+// Link hot starts use {lex site, mu, 1}; the event chains' R refreshes use
+// their own counters.
+__device__ __forceinline__ M3<double> haar_su3_philox(uint32_t key0, uint32_t key1, uint32_t c0, uint32_t c1,
+                                                      uint32_t c2) {
+    M3<double> q;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        U4 o = philox4x32_10(U4{c0, c1, c2, (uint32_t)k}, key0, key1);
+        double u1 = u53(o.x, o.y), u2 = u53(o.z, o.w);
+        double r = sqrt(-2.0 * log(u1));
+        double sn, cs;
+        sincos(2.0 * M_PI * u2, &sn, &cs);
+        q.v[2 * k] = r * cs;
+        q.v[2 * k + 1] = r * sn;
+    }
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+#pragma unroll
+        for (int pc = 0; pc < col; ++pc) {
+            double dr = 0, di = 0;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                dr += q.re(r, pc) * q.re(r, col) + q.im(r, pc) * q.im(r, col);
+                di += q.re(r, pc) * q.im(r, col) - q.im(r, pc) * q.re(r, col);
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                double a = q.re(r, pc) * dr - q.im(r, pc) * di;
+                double b = q.re(r, pc) * di + q.im(r, pc) * dr;
+                q.re(r, col) -= a;
+                q.im(r, col) -= b;
+            }
+        }
+        double n = 0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) n += q.re(r, col) * q.re(r, col) + q.im(r, col) * q.im(r, col);
+        n = 1.0 / sqrt(n);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) { q.re(r, col) *= n; q.im(r, col) *= n; }
+    }
+    // det by cofactors along row 0
+    double c0r = (q.re(1, 1) * q.re(2, 2) - q.im(1, 1) * q.im(2, 2)) - (q.re(1, 2) * q.re(2, 1) - q.im(1, 2) * q.im(2, 1));
+    double c0i = (q.re(1, 1) * q.im(2, 2) + q.im(1, 1) * q.re(2, 2)) - (q.re(1, 2) * q.im(2, 1) + q.im(1, 2) * q.re(2, 1));
+    double c1r = (q.re(1, 2) * q.re(2, 0) - q.im(1, 2) * q.im(2, 0)) - (q.re(1, 0) * q.re(2, 2) - q.im(1, 0) * q.im(2, 2));
+    double c1i = (q.re(1, 2) * q.im(2, 0) + q.im(1, 2) * q.re(2, 0)) - (q.re(1, 0) * q.im(2, 2) + q.im(1, 0) * q.re(2, 2));
+    double c2r = (q.re(1, 0) * q.re(2, 1) - q.im(1, 0) * q.im(2, 1)) - (q.re(1, 1) * q.re(2, 0) - q.im(1, 1) * q.im(2, 0));
+    double c2i = (q.re(1, 0) * q.im(2, 1) + q.im(1, 0) * q.re(2, 1)) - (q.re(1, 1) * q.im(2, 0) + q.im(1, 1) * q.re(2, 0));
+    double dre = q.re(0, 0) * c0r - q.im(0, 0) * c0i + q.re(0, 1) * c1r - q.im(0, 1) * c1i + q.re(0, 2) * c2r - q.im(0, 2) * c2i;
+    double dim = q.re(0, 0) * c0i + q.im(0, 0) * c0r + q.re(0, 1) * c1i + q.im(0, 1) * c1r + q.re(0, 2) * c2i + q.im(0, 2) * c2r;
+    double ph = -atan2(dim, dre) / 3.0;
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    M3<double> out;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        out.v[2 * k] = q.v[2 * k] * cs - q.v[2 * k + 1] * sn;
+        out.v[2 * k + 1] = q.v[2 * k] * sn + q.v[2 * k + 1] * cs;
+    }
+    return out;
 }
 
 }  // namespace qcb

@@ -35,6 +35,25 @@ class QcbError(RuntimeError):
 
 _lib = None
 
+
+class EcmcParams(C.Structure):
+    """qcb_ecmc_params (include/qcb.h); defaults of src/io/params.h:9-17."""
+    _fields_ = [("beta", C.c_double), ("theta_sample", C.c_double), ("theta_refresh_site", C.c_double),
+                ("theta_refresh_R", C.c_double), ("poisson", C.c_int), ("epsilon_set", C.c_double),
+                ("block", C.c_int * 4), ("seed", C.c_uint64), ("max_events", C.c_uint64)]
+
+
+def ecmc_params(beta=6.0, theta_sample=100.0, theta_refresh_site=50.0, theta_refresh_R=15.0, poisson=False,
+                epsilon_set=0.15, block=(4, 4, 4, 4), seed=1, max_events=0) -> EcmcParams:
+    return EcmcParams(beta, theta_sample, theta_refresh_site, theta_refresh_R, int(poisson), epsilon_set,
+                      (C.c_int * 4)(*block), seed, max_events)
+
+
+ECMC_CHAIN_DOUBLES = 32
+ECMC_FIELDS = [f"R{k}" for k in range(18)] + [
+    "th_refresh_site", "th_refresh_R", "par_refresh_site", "par_refresh_R", "events", "lifts", "revs",
+    "set_counter", "set_gen", "rng_n", "site", "mu", "eps", "initialized"]
+
 # (name, restype, argtypes) of every entry point declared in include/qcb.h
 SIGNATURES = [
     ("qcb_ctx_create", C.c_int, [C.c_int, C.POINTER(_VP)]),
@@ -92,6 +111,11 @@ SIGNATURES = [
     ("qcb_ildg_read", C.c_int, [C.c_char_p, C.POINTER(C.c_int), _D]),
     ("qcb_ildg_save", C.c_int, [_VP, C.c_char_p]),
     ("qcb_ildg_load", C.c_int, [_VP, C.c_char_p]),
+    ("qcb_ecmc_create", C.c_int, [_VP, C.POINTER(EcmcParams), C.POINTER(_VP)]),
+    ("qcb_ecmc_destroy", C.c_int, [_VP]),
+    ("qcb_ecmc_n_chains", C.c_int, [_VP, C.POINTER(C.c_int)]),
+    ("qcb_ecmc_sample", C.c_int, [_VP, C.c_int]),
+    ("qcb_ecmc_get_chains", C.c_int, [_VP, _D]),
     ("qcb_math_probe", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_int, _D]),
 ]
 
@@ -371,6 +395,40 @@ class Batch:
 
     def graph_launch(self, n_replays=1):
         self._c(lib().qcb_batch_graph_launch(self.h, n_replays))
+
+
+class Ecmc:
+    """Event chains, one per block of an FP64 lattice (qcb_ecmc_*, include/qcb.h)."""
+
+    def __init__(self, lat: Lattice, params: EcmcParams):
+        self.lat = lat
+        self.ctx = lat.ctx
+        self.h = _VP()
+        self.ctx.check(lib().qcb_ecmc_create(lat.h, C.byref(params), C.byref(self.h)))
+        n = C.c_int()
+        self.ctx.check(lib().qcb_ecmc_n_chains(self.h, C.byref(n)))
+        self.n_chains = n.value
+        self.ctx._children.add(self)
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().qcb_ecmc_destroy(self.h)
+        self.h = _VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sample(self, n=1):
+        self.ctx.check(lib().qcb_ecmc_sample(self.h, n))
+
+    def chains(self) -> np.ndarray:
+        """(n_chains, 32) chain states; columns named by ECMC_FIELDS."""
+        out = np.zeros((self.n_chains, ECMC_CHAIN_DOUBLES))
+        self.ctx.check(lib().qcb_ecmc_get_chains(self.h, _dp(out)))
+        return out
 
 
 class Service:

@@ -206,6 +206,44 @@ int qcb_ildg_read(const char* path, const int dims[4], double* aos);
 int qcb_ildg_save(qcb_lattice* lat, const char* path);
 int qcb_ildg_load(qcb_lattice* lat, const char* path);
 
+/* ---- event-chain Monte Carlo (src/ecmc/ecmc_mpi_cb.cpp, src/bin/gauge_ecmc_ncb.cpp) ----
+ * The lattice (FP64) is cut into blocks of block[0..3] sites, the reference's
+ * per-rank L_core^4 block with its c = 0 faces frozen
+ * (src/geometry/GeometryCB.cpp:33-56): link (x, mu) is frozen when
+ * x_d mod block_d == 0 for some d != mu.  One chain per block, all chains
+ * at once on the device; each keeps its LocalChainState
+ * (src/ecmc/ecmc_mpi_cb.h:15-35) between samples, as sample_persistant does
+ * across calls.  Philox4x32-10 counters, key = seed, per chain c and event
+ * index n: ctr = {c, n_lo, 0xEC000000 | n_hi, call} (call 0, 1: six
+ * gammas, lift select and accept; 2: set index; 3, 4: refresh site, dir,
+ * epsilon and its Poisson budget; 5: refresh-R budget; 6: theta_sample),
+ * Haar R refreshes on {c, n_lo, 0xED000000 | n_hi, 0..8}, matrix-set element
+ * 2m+1 of generation g on {c, g, 0xE5000000 | m, 0..2}. */
+typedef struct qcb_ecmc qcb_ecmc;
+typedef struct {
+    double beta;
+    double theta_sample;        /* param_theta_sample (src/io/params.h:9-17) */
+    double theta_refresh_site;  /* param_theta_refresh_site */
+    double theta_refresh_R;     /* param_theta_refresh_R */
+    int poisson;                /* exponential budgets instead of fixed ones */
+    double epsilon_set;         /* spread of the near-identity matrix set */
+    int block[4];               /* chain block extents (>= 4, dividing the lattice) */
+    uint64_t seed;
+    uint64_t max_events;        /* per chain and sample call; 0 = 1e9.  Reaching it ends that
+                                   chain's sample early and qcb_ecmc_get_chains reports QCB_EBUDGET */
+} qcb_ecmc_params;
+#define QCB_ECMC_CHAIN_DOUBLES 32 /* R[18], th_refresh_site, th_refresh_R, par_refresh_site,
+                                     par_refresh_R, events, lifts, revs, set_counter, set_gen,
+                                     rng_n, site (lex), mu, epsilon, initialized */
+/* mpi::ecmccb::sample_persistant chain states for every block of lat */
+int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_ecmc** out);
+int qcb_ecmc_destroy(qcb_ecmc* e);
+int qcb_ecmc_n_chains(qcb_ecmc* e, int* n);
+/* n_samples calls of sample_persistant on every chain (src/ecmc/ecmc_mpi_cb.cpp:496-637) */
+int qcb_ecmc_sample(qcb_ecmc* e, int n_samples);
+/* chain states, QCB_ECMC_CHAIN_DOUBLES doubles per chain (synchronises) */
+int qcb_ecmc_get_chains(qcb_ecmc* e, double* out);
+
 /* ---- device math probes (checking aids for the register-level routines) ---- */
 #define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out; in is
                                   traceless anti-Hermitian: its upper triangle and Im diagonal are read */

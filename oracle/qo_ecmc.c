@@ -498,6 +498,7 @@ static void sample_chain(qo_lat* l, const qo_ecmc_params* p, const int org[4], q
                 j = i;
             }
         }
+        if (!(theta_reject >= 0.0)) break;   /* non-finite angle: stop instead of looping */
         double d_sample = th_sample - par_sample;
         double d_rs = st->th_refresh_site - st->par_refresh_site;
         double d_rR = st->th_refresh_R - st->par_refresh_R;

@@ -1,0 +1,106 @@
+"""Event chains on the GPU (reference src/ecmc/ecmc_mpi_cb.cpp:12-637) against
+the oracle's restatement on the same Philox counters: links and chain states
+after a sample, frozen links untouched, SU(3) kept, and the equilibrium
+plaquette of the chains against the reference-method fixture."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as qo
+import qcb
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+F = {n: k for k, n in enumerate(qcb.ECMC_FIELDS)}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qcb.Context(0)
+    yield c
+    c.close()
+
+
+def _pair(ctx, dims, seed):
+    g = qcb.Lattice(ctx, dims)
+    g.hot(seed)
+    o = qo.Lattice(dims)
+    o.set_links(g.get_links())
+    return g, o
+
+
+def _compare(chains_gpu, chains_or):
+    for c, s in zip(chains_gpu, chains_or):
+        assert int(c[F["site"]]) == s.site and int(c[F["mu"]]) == s.mu and int(c[F["eps"]]) == s.eps
+        assert int(c[F["events"]]) == s.events and int(c[F["lifts"]]) == s.lifts and int(c[F["revs"]]) == s.revs
+        assert int(c[F["set_gen"]]) == s.set_gen and int(c[F["rng_n"]]) == s.rng_n
+        assert np.abs(c[:18] - np.array(s.R[:])).max() < 1e-9
+        assert abs(c[F["par_refresh_site"]] - s.par_refresh_site) < 1e-8
+        assert abs(c[F["par_refresh_R"]] - s.par_refresh_R) < 1e-8
+        assert abs(c[F["th_refresh_site"]] - s.th_refresh_site) < 1e-8
+
+
+@pytest.mark.parametrize("poisson,block", [(False, (4, 4, 4, 4)), (True, (4, 4, 8, 4))])
+def test_chains_match_oracle(ctx, poisson, block):
+    dims = (8, 8, 8, 8)
+    g, o = _pair(ctx, dims, 3)
+    # budgets with no common multiple: with fixed budgets 12 and 3 the sample
+    # event and a refresh-R event coincide exactly, and rounding picks one
+    kw = dict(beta=6.0, theta_sample=12.3, theta_refresh_site=5.1, theta_refresh_R=3.7, poisson=poisson,
+              block=block)
+    e = qcb.Ecmc(g, qcb.ecmc_params(seed=77, max_events=200000, **kw))
+    ch = o.ecmc_chains(qo.ecmc_params(**kw))
+    e.sample(1)
+    assert o.ecmc_philox(qo.ecmc_params(**kw), ch, 77) == e.n_chains
+    gc = e.chains()
+    _compare(gc, ch)
+    assert np.abs(g.get_links() - o.links).max() < 1e-10
+    # a second sample continues the same chains
+    g.set_links(o.links)
+    e.sample(1)
+    o.ecmc_philox(qo.ecmc_params(**kw), ch, 77)
+    _compare(e.chains(), ch)
+    assert np.abs(g.get_links() - o.links).max() < 1e-9
+    e.close()
+
+
+def test_frozen_links_untouched_and_su3(ctx):
+    dims = (8, 8, 8, 12)
+    g = qcb.Lattice(ctx, dims)
+    g.hot(5)
+    g.hb_sweeps(6.0, 2, 5, seed=5)
+    U0 = g.get_links()
+    e = qcb.Ecmc(g, qcb.ecmc_params(seed=3, block=(4, 4, 4, 4), max_events=200000))
+    e.sample(4)
+    U1 = g.get_links()
+    c = np.indices(dims[::-1]).reshape(4, -1)[::-1]
+    fr = np.zeros((c.shape[1], 4), dtype=bool)
+    for mu in range(4):
+        for d in range(4):
+            if d != mu:
+                fr[:, mu] |= (c[d] % 4) == 0
+    assert np.array_equal(U1[fr], U0[fr])
+    assert (np.abs(U1[~fr] - U0[~fr]).max(axis=(1, 2)) > 0).mean() > 0.3
+    U = U1.reshape(-1, 3, 3)
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-12
+    ch = e.chains()
+    assert np.all(ch[:, F["initialized"]] == 1) and np.all(ch[:, F["events"]] >= ch[:, F["lifts"]])
+    e.close()
+
+
+def test_event_budget_is_reported(ctx):
+    g = qcb.Lattice(ctx, (8, 8, 8, 8))
+    g.hot(1)
+    e = qcb.Ecmc(g, qcb.ecmc_params(seed=2, theta_sample=1e6, max_events=50))
+    e.sample(1)
+    with pytest.raises(qcb.QcbError):
+        e.chains()
+    e.close()
+    with pytest.raises(qcb.QcbError):
+        qcb.Ecmc(g, qcb.ecmc_params(block=(3, 4, 4, 4)))
+    f = qcb.Lattice(ctx, (8, 8, 8, 8), qcb.F32)
+    with pytest.raises(qcb.QcbError):
+        qcb.Ecmc(f, qcb.ecmc_params())
