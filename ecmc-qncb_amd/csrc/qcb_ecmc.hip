@@ -7,7 +7,7 @@
 // and is cut into the same blocks: a link (x, mu) is frozen when
 // x_d mod b_d == 0 for some d != mu, a non-frozen link of a block only reads
 // links of its own block or frozen ones, so every block's chain runs at once,
-// one chain per thread.  Chains keep their LocalChainState between samples
+// one chain per team of 8 lanes.  Chains keep their LocalChainState between samples
 // (sample_persistant) in device memory.  Randomness is Philox4x32-10 on
 // per-chain counters (scheme in include/qcb.h), so each chain's trajectory
 // depends only on (seed, chain index, its own history).  FP64 links.
@@ -46,7 +46,11 @@ struct EcmcArgs {
 };
 
 constexpr uint64_t kSetSize = 10001;   // ecmc_set (src/su3/utils.cpp:119-127), 1 + 5000 pairs
-constexpr int kChainsPerBlock = 64;
+// A chain is worked by a team of 8 lanes: lanes 0-5 form and solve the six
+// staples of an event in parallel, lanes 0-3 the four plaquettes of a lift;
+// the rest of the event is computed identically by every lane of the team.
+constexpr int kTeam = 8;
+constexpr int kThreads = 64;   // one wavefront per block: 8 chains
 
 __device__ inline int dsgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
 
@@ -268,10 +272,11 @@ __device__ M3<double> plaq_P(int i, bool fwd, const M3<double>& U0, const M3<dou
     return dag(mul_nn(U3, T));
 }
 
-__global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<double>* __restrict__ lat,
-                                                                 EcmcChainDev* __restrict__ chains, EcmcArgs a) {
-    const int cid = blockIdx.x * kChainsPerBlock + threadIdx.x;
-    if (cid >= a.n_chains) return;
+__global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* __restrict__ lat,
+                                                          EcmcChainDev* __restrict__ chains, EcmcArgs a) {
+    const int tid = blockIdx.x * kThreads + threadIdx.x;
+    const int cid = tid / kTeam, lane = tid % kTeam;
+    if (cid >= a.n_chains) return;   // whole teams leave together
     const LatDev<double>& L = *lat;
     const uint32_t chain = (uint32_t)cid;
     EcmcChainDev st = chains[cid];
@@ -334,9 +339,12 @@ __global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<do
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
         const M3<double> Ucur = ldl(L, a, st.site, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
-        int jmin = 0;
-        double theta_reject = 0.0;
-        for (int j = 0; j < 6; ++j) {
+        // lane j < 6 solves staple j; the team takes the smallest angle, the
+        // lowest j on ties (the sequential scan's first minimum)
+        double theta_reject = __builtin_inf();
+        int jmin = lane;
+        if (lane < 6) {
+            const int j = lane;
             const M3<double> S = staple_j(L, a, st.site, st.mu, j);
             double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
 #pragma unroll
@@ -354,10 +362,18 @@ __global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<do
                 p11r += m1r * R.re(col, 1) - m1i * R.im(col, 1);
                 p11i += m1r * R.im(col, 1) + m1i * R.re(col, 1);
             }
-            const double rej = reject_fast((p00r + p11r) * beta_red, (p11i - p00i) * beta_red, gam[j], st.eps);
-            if (j == 0 || rej < theta_reject) {
-                theta_reject = rej;
-                jmin = j;
+            const double gj = j == 0 ? gam[0] : j == 1 ? gam[1] : j == 2 ? gam[2] : j == 3 ? gam[3] : j == 4 ? gam[4] : gam[5];
+            theta_reject = reject_fast((p00r + p11r) * beta_red, (p11i - p00i) * beta_red, gj, st.eps);
+            // a NaN angle must win the reduction so the team stops
+            if (!(theta_reject >= 0.0)) theta_reject = -1.0;
+        }
+#pragma unroll
+        for (int off = kTeam / 2; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(theta_reject, off, kTeam);
+            const int oj = __shfl_xor(jmin, off, kTeam);
+            if (ov < theta_reject || (ov == theta_reject && oj < jmin)) {
+                theta_reject = ov;
+                jmin = oj;
             }
         }
         if (!(theta_reject >= 0.0)) {   // a non-finite angle would never reach the sample event
@@ -406,10 +422,20 @@ __global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<do
             plaq_links(a, st.site, st.mu, jmin, ls, ld);
             const M3<double> U1 = ldl(L, a, ls[1], ld[1]), U2 = ldl(L, a, ls[2], ld[2]), U3 = ldl(L, a, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
+            // lane i < 4 forms plaquette i; every lane then holds all four dS
+            double my_ds = 0.0;
+            if (lane < 4) {
+                const int i = lane;
+                const uint32_t si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
+                const int di = i == 0 ? ld[0] : i == 1 ? ld[1] : i == 2 ? ld[2] : ld[3];
+                my_ds = frozen(a, si, di) ? 0.0 : compute_ds(plaq_P(i, fwd, U0, U1, U2, U3), R);
+            }
+            const int base = (threadIdx.x / kTeam) * kTeam;
             double absd[4], sum = 0.0;
             int sgn[4];
+#pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const double ds = frozen(a, ls[i], ld[i]) ? 0.0 : compute_ds(plaq_P(i, fwd, U0, U1, U2, U3), R);
+                const double ds = __shfl(my_ds, base + i, 64);
                 sgn[i] = dsgn(ds);
                 absd[i] = fabs(ds);
                 sum += absd[i];
@@ -442,7 +468,7 @@ __global__ __launch_bounds__(kChainsPerBlock) void k_ecmc_sample(const LatDev<do
     st.set_gen = gen;
 #pragma unroll
     for (int k = 0; k < 18; ++k) st.R[k] = R.v[k];
-    chains[cid] = st;
+    if (lane == 0) chains[cid] = st;
 }
 
 }  // namespace
@@ -519,9 +545,9 @@ extern "C" int qcb_ecmc_sample(qcb_ecmc* e, int n_samples) {
     if (!e || n_samples < 0) return QCB_EINVAL;
     qcb_lattice* lat = e->lat;
     qcb_ctx* ctx = lat->ctx;
-    const int blocks = (e->n_chains + kChainsPerBlock - 1) / kChainsPerBlock;
+    const int blocks = (e->n_chains * kTeam + kThreads - 1) / kThreads;
     for (int s = 0; s < n_samples; ++s)
-        hipLaunchKernelGGL(k_ecmc_sample, dim3(blocks), dim3(kChainsPerBlock), 0, ctx->stream,
+        hipLaunchKernelGGL(k_ecmc_sample, dim3(blocks), dim3(kThreads), 0, ctx->stream,
                            (const LatDev<double>*)lat->d_desc, e->d_chains, e->args);
     QCB_HIP(ctx, hipGetLastError());
     return QCB_OK;
