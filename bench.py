@@ -162,6 +162,54 @@ def serve(qcb, ctx, n_req=60, rate=20.0, n_therm=50, seed=2026):
                 mean_sites=sites / n_req)
 
 
+def ecmc(qcb, ctx, L=32, block=4, samples=2):
+    """Event chains (FP64) on an L^4 lattice cut into block^4 chain blocks,
+    the reference's ECMC input-template budgets (theta_sample 1800,
+    theta_refresh_site 1500, theta_refresh_R 150, epsilon_set 0.15) at
+    beta = 6.0, a random shift before every sample; events/s over all chains."""
+    import numpy as np
+    lat = qcb.Lattice(ctx, (L,) * 4)
+    lat.hot(5)
+    lat.hb_sweeps(6.0, 2, 50, seed=5)
+    e = qcb.Ecmc(lat, qcb.ecmc_params(beta=6.0, theta_sample=1800.0, theta_refresh_site=1500.0,
+                                      theta_refresh_R=150.0, epsilon_set=0.15, block=(block,) * 4, seed=5))
+    rng = np.random.default_rng(5)
+    lat.translate(rng.integers(0, L, size=4))
+    e.sample(1)
+    ev = qcb.ECMC_FIELDS.index("events")
+    n0 = e.chains()[:, ev].sum()
+    ctx.timing_begin()
+    for _ in range(samples):
+        lat.translate(rng.integers(0, L, size=4))
+        e.sample(1)
+    ms = ctx.timing_end()
+    n1 = e.chains()[:, ev].sum()
+    out = dict(lattice=[L] * 4, block=[block] * 4, chains=e.n_chains, events_per_s=(n1 - n0) / (ms * 1e-3),
+               ms_per_sample=ms / samples, events_per_chain_per_sample=(n1 - n0) / samples / e.n_chains,
+               plaquette=lat.plaquette())
+    e.close()
+    lat.close()
+    return out
+
+
+def ecmc_cpu_baseline(L=8):
+    """The reference's one-rank event chain (whole L^4 lattice one block,
+    mt19937_64, FP64) restated by the oracle, one core: events/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as qo
+    lat = qo.Lattice((L,) * 4)
+    lat.hot_philox(5)
+    p = qo.ecmc_params(beta=6.0, theta_sample=1800.0, theta_refresh_site=1500.0, theta_refresh_R=150.0)
+    ch = qo.EcmcChain()
+    mt = qo.Mt64(12345)
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 3.0:
+        lat.ecmc_ref(p, ch, mt)
+    dt = time.perf_counter() - t0
+    return dict(value=ch.events / dt, unit="events/s", cores=1, kind="port",
+                sample=f"reference-order event chain on a {L}^4 one-rank lattice: {ch.events} events in {dt:.1f} s")
+
+
 def precision_report(qcb, ctx, L=32, sweeps=500, flow_steps=150, seed=11, beta=6.0):
     """FP32 links (reunitarised every 4 sweeps) against the FP64 GPU run on
     identical Philox uniforms: |<P>| differences along the chain, then the
@@ -250,6 +298,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-ecmc", action="store_true")
     ap.add_argument("--precision-report", action="store_true",
                     help="run the FP32-vs-FP64 comparison (32^4, 500 sweeps, 150 flow steps) and print it")
     ap.add_argument("--serve-requests", type=int, default=60)
@@ -294,6 +343,11 @@ def main():
         bt = batch64(qcb, ctx)
         log(f"[bench] batch64 {bt['link_updates_per_s']:.3e} link-updates/s")
         comp["batch64_fp32"] = bt
+    if not args.no_ecmc:
+        log(f"[bench] event chains, {L}^4 in 4^4 blocks")
+        ec = ecmc(qcb, ctx, L)
+        log(f"[bench] ECMC {ec['events_per_s']:.3e} events/s ({ec['chains']} chains)")
+        comp["ecmc_fp64"] = ec
     if not args.no_serve:
         log(f"[bench] flow service: {args.serve_requests} Poisson requests at 20/s")
         sv = serve(qcb, ctx, args.serve_requests)
@@ -306,6 +360,9 @@ def main():
         log("[bench] CPU baseline (reference-order FP64 sweep, 1 core)")
         cpu = cpu_baseline(L)
         log(f"[bench] CPU {cpu['value']:.3e} link-updates/s")
+        if not args.no_ecmc:
+            comp["ecmc_cpu_baseline"] = ecmc_cpu_baseline()
+            log(f"[bench] CPU ECMC {comp['ecmc_cpu_baseline']['value']:.3e} events/s")
 
     V = L ** 4
     out = {
