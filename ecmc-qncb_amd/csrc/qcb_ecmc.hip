@@ -577,3 +577,46 @@ extern "C" int qcb_ecmc_get_chains(qcb_ecmc* e, double* out) {
                              "rejection angle)", bad);
     return QCB_OK;
 }
+
+extern "C" int qcb_ecmc_set_chains(qcb_ecmc* e, const double* in) {
+    if (!e || !in) return QCB_EINVAL;
+    qcb_ctx* ctx = e->lat->ctx;
+    const EcmcArgs& a = e->args;
+    std::vector<EcmcChainDev> h(e->n_chains);
+    for (int c = 0; c < e->n_chains; ++c) {
+        const double* v = in + (size_t)c * QCB_ECMC_CHAIN_DOUBLES;
+        EcmcChainDev& s = h[c];
+        std::memset(&s, 0, sizeof s);
+        for (int k = 0; k < 18; ++k) s.R[k] = v[k];
+        s.th_rs = v[18]; s.th_rR = v[19]; s.par_rs = v[20]; s.par_rR = v[21];
+        s.events = (unsigned long long)v[22]; s.lifts = (unsigned long long)v[23]; s.revs = (unsigned long long)v[24];
+        s.set_counter = (unsigned long long)v[25]; s.set_gen = (unsigned long long)v[26];
+        s.rng_n = (unsigned long long)v[27];
+        s.mu = (int)v[29]; s.eps = (int)v[30]; s.initialized = v[31] != 0.0;
+        // the site must lie in the chain's own block (or be unset for a fresh chain)
+        const double site = v[28];
+        const uint64_t V = (uint64_t)a.L[0] * a.L[1] * a.L[2] * a.L[3];
+        if (s.initialized) {
+            if (!(site >= 0 && site < (double)V) || s.mu < 0 || s.mu > 3 || (s.eps != 1 && s.eps != -1))
+                return qcb_set_error(ctx, QCB_EINVAL, "chain %d: site/mu/epsilon out of range", c);
+            uint32_t x = (uint32_t)site;
+            int r = c;
+            for (int d = 0; d < 4; ++d) {
+                const int bc = r % a.nb[d];
+                r /= a.nb[d];
+                const int xd = (int)(x % (uint32_t)a.L[d]);
+                x /= (uint32_t)a.L[d];
+                if (xd / a.b[d] != bc)
+                    return qcb_set_error(ctx, QCB_EINVAL, "chain %d: site outside its block", c);
+                // the active link must be one the chain may move (not on a frozen face)
+                if (d != s.mu && xd % a.b[d] == 0)
+                    return qcb_set_error(ctx, QCB_EINVAL, "chain %d: active link is frozen", c);
+            }
+            s.site = (uint32_t)site;
+        }
+    }
+    QCB_HIP(ctx, hipMemcpyAsync(e->d_chains, h.data(), sizeof(EcmcChainDev) * h.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QCB_OK;
+}

@@ -23,13 +23,12 @@ namespace io = qcb_host::io;
 
 static void generate_hb_cb(const RunParamsHbCB& rp, bool existing) {
     mpi::MpiTopology topo(rp.n_core_dims);
-    const int L = rp.L_core * rp.n_core_dims;
-    GeometryCB geo(L);
+    GeometryCB geo(rp.L_core, rp.n_core_dims);
     const int prec = rp.precision == "fp32" ? QCB_F32 : QCB_F64;
     const int reunit = rp.reunit_every >= 0 ? rp.reunit_every : (prec == QCB_F32 ? 4 : 0);
     GaugeField field(geo, prec, reunit);
     std::mt19937_64 rng(rp.seed + topo.rank);
-    if (!rp.cold_start) field.hot_start(geo, rng);
+    if (!rp.cold_start) field.hot_start(geo, rng, rp.seed);
 
     if (existing) {
         read_ildg_clime(rp.run_name, rp.run_dir, field, geo, topo);

@@ -48,12 +48,28 @@ struct MpiTopology {
 };
 }  // namespace mpi
 
-// Lattice extents (GeometryCB, src/geometry/GeometryCB.h): no tables.
+// Lattice extents (GeometryCB, src/geometry/GeometryCB.h): no tables.  The
+// reference's GeometryCB is one rank's L_core^4 block of an n_core^4 Cartesian
+// grid of ranks; here L is the whole lattice and Lc the block, which bounds
+// the random shifts (src/mpi/Shift.cpp:166) and the hot start's rank blocks.
 struct GeometryCB {
     int L[4];
-    explicit GeometryCB(int L_) : L{L_, L_, L_, L_} {}
-    GeometryCB(int lx, int ly, int lz, int lt) : L{lx, ly, lz, lt} {}
+    int Lc[4];
+    int n_core = 1;
+    explicit GeometryCB(int L_) : L{L_, L_, L_, L_}, Lc{L_, L_, L_, L_} {}
+    GeometryCB(int L_core, int n_core_dims)
+        : L{L_core * n_core_dims, L_core * n_core_dims, L_core * n_core_dims, L_core * n_core_dims},
+          Lc{L_core, L_core, L_core, L_core}, n_core(n_core_dims) {}
+    GeometryCB(int lx, int ly, int lz, int lt) : L{lx, ly, lz, lt}, Lc{lx, ly, lz, lt} {}
     size_t V() const { return (size_t)L[0] * L[1] * L[2] * L[3]; }
+    int n_ranks() const { return n_core * n_core * n_core * n_core; }
+    // block coordinates of a rank in MPI_Cart_create's row-major order (t fastest)
+    void rank_block(int r, int b[4]) const {
+        for (int d = 3; d >= 0; --d) {
+            b[d] = r % n_core;
+            r /= n_core;
+        }
+    }
 };
 
 // SU(3) draw of the reference's hot start (src/su3/utils.cpp:13-30) on the
@@ -80,11 +96,27 @@ public:
     GaugeField(const GaugeField&) = delete;
     GaugeField& operator=(const GaugeField&) = delete;
 
-    // GaugeField::hot_start (src/gauge/GaugeField.cpp:10-20): host draws from
-    // the caller's mt19937_64 in t,z,y,x,mu order, then one upload.
-    void hot_start(const GeometryCB& g, std::mt19937_64& rng) {
+    // GaugeField::hot_start (src/gauge/GaugeField.cpp:10-20): host draws in
+    // t,z,y,x,mu order over each rank's block, then one upload.  Rank 0 draws
+    // from the caller's mt19937_64 (seeded seed + 0); rank r > 0 from its own
+    // mt19937_64(seed + r), as each reference rank seeds its generator.
+    void hot_start(const GeometryCB& g, std::mt19937_64& rng, int seed = 0) {
         std::vector<double> h(g.V() * 72);
-        for (size_t k = 0; k < g.V() * 4; ++k) random_su3(rng, h.data() + 18 * k);
+        for (int r = 0; r < g.n_ranks(); ++r) {
+            std::mt19937_64 own((uint64_t)(seed + r));
+            std::mt19937_64& gen = r == 0 ? rng : own;
+            int b[4];
+            g.rank_block(r, b);
+            for (int t = 0; t < g.Lc[3]; ++t)
+                for (int z = 0; z < g.Lc[2]; ++z)
+                    for (int y = 0; y < g.Lc[1]; ++y)
+                        for (int x = 0; x < g.Lc[0]; ++x) {
+                            const size_t X = b[0] * g.Lc[0] + x, Y = b[1] * g.Lc[1] + y, Z = b[2] * g.Lc[2] + z,
+                                         T = b[3] * g.Lc[3] + t;
+                            const size_t site = ((T * g.L[2] + Z) * g.L[1] + Y) * g.L[0] + X;
+                            for (int mu = 0; mu < 4; ++mu) random_su3(gen, h.data() + 18 * (4 * site + mu));
+                        }
+        }
         check(qcb_lattice_set_links(lat, h.data()), "qcb_lattice_set_links");
     }
     void cold_start(const GeometryCB&) { check(qcb_cold(lat), "qcb_cold"); }
@@ -170,7 +202,7 @@ namespace shift {
 inline void random_shift(GaugeField& field, const GeometryCB& geo, HalosShift&, MpiTopology& topo, std::mt19937_64& rng) {
     int s[4];
     for (int d = 0; d < 4; ++d) {
-        std::uniform_int_distribution<int> rand_l_shift(0, geo.L[d] - 1);
+        std::uniform_int_distribution<int> rand_l_shift(0, geo.Lc[d] - 1);
         s[d] = rand_l_shift(rng);
     }
     if (topo.rank == 0) std::printf("Shift (%d, %d, %d, %d)\n", s[0], s[1], s[2], s[3]);

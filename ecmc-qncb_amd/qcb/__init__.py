@@ -116,6 +116,7 @@ SIGNATURES = [
     ("qcb_ecmc_n_chains", C.c_int, [_VP, C.POINTER(C.c_int)]),
     ("qcb_ecmc_sample", C.c_int, [_VP, C.c_int]),
     ("qcb_ecmc_get_chains", C.c_int, [_VP, _D]),
+    ("qcb_ecmc_set_chains", C.c_int, [_VP, _D]),
     ("qcb_math_probe", C.c_int, [_VP, C.c_int, C.c_int, C.c_int, _D, C.c_double, C.c_int, _D]),
 ]
 
@@ -423,6 +424,11 @@ class Ecmc:
 
     def sample(self, n=1):
         self.ctx.check(lib().qcb_ecmc_sample(self.h, n))
+
+    def set_chains(self, states: np.ndarray):
+        st = np.ascontiguousarray(states, dtype=np.float64)
+        assert st.shape == (self.n_chains, ECMC_CHAIN_DOUBLES)
+        self.ctx.check(lib().qcb_ecmc_set_chains(self.h, _dp(st)))
 
     def chains(self) -> np.ndarray:
         """(n_chains, 32) chain states; columns named by ECMC_FIELDS."""

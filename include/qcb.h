@@ -235,7 +235,8 @@ typedef struct {
 #define QCB_ECMC_CHAIN_DOUBLES 32 /* R[18], th_refresh_site, th_refresh_R, par_refresh_site,
                                      par_refresh_R, events, lifts, revs, set_counter, set_gen,
                                      rng_n, site (lex), mu, epsilon, initialized */
-/* mpi::ecmccb::sample_persistant chain states for every block of lat */
+/* mpi::ecmccb::sample_persistant chain states for every block of lat (chain c
+ * is block c in x-fastest block order); lat must outlive the handle */
 int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_ecmc** out);
 int qcb_ecmc_destroy(qcb_ecmc* e);
 int qcb_ecmc_n_chains(qcb_ecmc* e, int* n);
@@ -243,6 +244,10 @@ int qcb_ecmc_n_chains(qcb_ecmc* e, int* n);
 int qcb_ecmc_sample(qcb_ecmc* e, int n_samples);
 /* chain states, QCB_ECMC_CHAIN_DOUBLES doubles per chain (synchronises) */
 int qcb_ecmc_get_chains(qcb_ecmc* e, double* out);
+/* restore chain states (io::load_state, src/io/io.cpp:200-232); an
+ * initialized chain's site must lie in its own block, its active link (site, mu)
+ * off the frozen faces and epsilon = +-1, else QCB_EINVAL and nothing changes */
+int qcb_ecmc_set_chains(qcb_ecmc* e, const double* in);
 
 /* ---- device math probes (checking aids for the register-level routines) ---- */
 #define QCB_PROBE_EXP 0        /* out = exp(param * in) (utils.cpp:145-193), 18 in / 18 out; in is

@@ -92,6 +92,7 @@ def lib():
             "qo_lat_init": (C.c_int, [P(QoLat), P(C.c_int)]),
             "qo_lat_free": (None, [P(QoLat)]),
             "qo_lat_cold": (None, [P(QoLat)]),
+            "qo_random_su3_ref": (None, [P(QoMt64), P(C.c_double)]),
             "qo_lat_hot_ref": (None, [P(QoLat), P(QoMt64)]),
             "qo_lat_hot_philox": (None, [P(QoLat), C.c_uint64]),
             "qo_hot_link_philox": (None, [C.c_uint64, C.c_uint32, C.c_uint32, _D]),
@@ -256,6 +257,12 @@ class Mt64:
 
     def next(self) -> int:
         return lib().qo_mt64_next(C.byref(self.s))
+
+    def random_su3(self) -> np.ndarray:
+        """one SU(3) draw of the reference's hot start (src/su3/utils.cpp:13-30)"""
+        out = np.zeros(18)
+        lib().qo_random_su3_ref(C.byref(self.s), out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out.view(np.complex128).reshape(3, 3)
 
     def canonical(self) -> float:
         return lib().qo_mt64_canonical(C.byref(self.s))

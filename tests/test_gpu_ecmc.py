@@ -104,6 +104,48 @@ def test_event_budget_is_reported(ctx):
         qcb.Ecmc(f, qcb.ecmc_params())
 
 
+def test_set_chains_round_trip_and_validation(ctx):
+    """Saved chain states restore exactly (io::load_state, src/io/io.cpp:200-232):
+    a chain restored into a fresh generator continues as the original does;
+    states that would move a link of another block or a frozen link are refused."""
+    dims = (8, 8, 8, 8)
+    kw = dict(theta_sample=12.3, theta_refresh_site=5.1, theta_refresh_R=3.7, seed=5, max_events=200000)
+    g = qcb.Lattice(ctx, dims)
+    g.hot(8)
+    e = qcb.Ecmc(g, qcb.ecmc_params(**kw))
+    e.sample(1)
+    st = e.chains()
+    U = g.get_links()
+    e.sample(1)
+    want_links, want_chains = g.get_links(), e.chains()
+    f = qcb.Lattice(ctx, dims)
+    f.set_links(U)
+    e2 = qcb.Ecmc(f, qcb.ecmc_params(**kw))
+    e2.set_chains(st)
+    assert np.array_equal(e2.chains(), st)
+    e2.sample(1)
+    assert np.array_equal(f.get_links(), want_links)
+    assert np.array_equal(e2.chains(), want_chains)
+    bad = st.copy()
+    bad[0, F["site"]] = st[1, F["site"]]          # another block's site
+    with pytest.raises(qcb.QcbError):
+        e2.set_chains(bad)
+    bad = st.copy()
+    mu = int(bad[0, F["mu"]])
+    x = [int(bad[0, F["site"]]) // 8 ** d % 8 for d in range(4)]
+    x[(mu + 1) % 4] = 0                            # on a frozen face for this mu
+    bad[0, F["site"]] = sum(v * 8 ** d for d, v in enumerate(x))
+    with pytest.raises(qcb.QcbError):
+        e2.set_chains(bad)
+    bad = st.copy()
+    bad[0, F["eps"]] = 0
+    with pytest.raises(qcb.QcbError):
+        e2.set_chains(bad)
+    assert np.array_equal(e2.chains(), want_chains)   # refused states change nothing
+    e.close()
+    e2.close()
+
+
 def _binned(x, nbin):
     x = np.asarray(x)
     bins = x[: len(x) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
