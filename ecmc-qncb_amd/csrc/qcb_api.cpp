@@ -155,8 +155,7 @@ extern "C" int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt, 
 
 extern "C" int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes) {
     if (!lat || !bytes) return QCB_EINVAL;
-    uint64_t f = field_bytes(lat);
-    *bytes = f * (1 + (lat->F[0] ? 3 : 0));
+    *bytes = field_bytes(lat) + (lat->F[0] ? flow_bytes(lat) : 0);
     return QCB_OK;
 }
 
@@ -269,14 +268,18 @@ extern "C" int qcb_clover_qe(qcb_lattice* lat, double* Q, double* E) {
 
 extern "C" int qcb_staples(qcb_lattice* lat, double* out) {
     if (!lat || !out) return QCB_EINVAL;
-    int rc = ensure_flow_buffers(lat);
-    if (rc) return rc;
+    qcb_ctx* ctx = lat->ctx;
+    size_t n = (size_t)lat->V * 72;
+    double* d = nullptr;
+    QCB_HIP(ctx, hipMallocAsync((void**)&d, n * sizeof(double), ctx->stream));
     Launch g = lat_launch(lat, kFamLink);
-    lat->flow_active = false;
-    if (lat->prec == QCB_F64) launch_staples<double>(g, (double*)lat->F[2]);
-    else launch_staples<float>(g, (float*)lat->F[2]);
-    QCB_HIP(lat->ctx, hipGetLastError());
-    return download_field(lat, lat->F[2], out);
+    if (lat->prec == QCB_F64) launch_staples<double>(g, d);
+    else launch_staples<float>(g, d);
+    QCB_HIP(ctx, hipGetLastError());
+    QCB_HIP(ctx, hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
+    QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return QCB_OK;
 }
 
 // ------------------------------------------------------------- heatbath ---

@@ -94,7 +94,7 @@ __device__ inline M3<double> ldl(const LatDev<double>& L, const EcmcArgs& a, uin
 __device__ inline void stl(const LatDev<double>& L, const EcmcArgs& a, uint32_t s, int mu, const M3<double>& m) {
     int c[4];
     gco(a, s, c);
-    double2* b = (double2*)L.U + link_base(L.S, mu * 2 + ((c[0] + c[1] + c[2] + c[3]) & 1), s >> 1);
+    double2* b = (double2*)L.U + link_base<double>(L.S, mu * 2 + ((c[0] + c[1] + c[2] + c[3]) & 1), s >> 1);
 #pragma unroll
     for (int k = 0; k < 9; ++k) b[64 * k] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
 }

@@ -54,7 +54,15 @@ int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
 
 namespace qcb {
 
-inline uint64_t field_bytes(const qcb_lattice* l) { return (uint64_t)8 * 18 * l->S * (uint64_t)l->esize; }
+// Bytes of one link field (8 half-site classes of S links): 18 reals per link
+// in FP64, 12 in FP32 (rows 0 and 1; the kernels rebuild row 2).
+inline uint64_t field_bytes(const qcb_lattice* l) {
+    return (uint64_t)8 * (l->esize == 4 ? 12 : 18) * l->S * (uint64_t)l->esize;
+}
+// Bytes of the flow's force accumulator: 5 complex slots (9 reals used) per link.
+inline uint64_t force_bytes(const qcb_lattice* l) { return (uint64_t)8 * 10 * l->S * (uint64_t)l->esize; }
+// Flow buffers: two link fields (W double buffer) and the force accumulator.
+inline uint64_t flow_bytes(const qcb_lattice* l) { return 2 * field_bytes(l) + force_bytes(l); }
 
 // Default work-order tile extent along one direction: largest divisor <= 4.
 inline int default_tile(int L) {
@@ -165,10 +173,12 @@ inline int lattice_alloc(qcb_lattice* l) {
 
 inline int ensure_flow_buffers(qcb_lattice* l) {
     if (l->F[0]) return QCB_OK;
-    for (int k = 0; k < 3; ++k)
-        if (hipMalloc(&l->F[k], field_bytes(l)) != hipSuccess)
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t n = k < 2 ? field_bytes(l) : force_bytes(l);
+        if (hipMalloc(&l->F[k], n) != hipSuccess)
             return qcb_set_error(l->ctx, QCB_ENOMEM, "cannot allocate flow buffer %d (%llu bytes)", k,
-                                 (unsigned long long)field_bytes(l));
+                                 (unsigned long long)n);
+    }
     return upload_desc(l);
 }
 

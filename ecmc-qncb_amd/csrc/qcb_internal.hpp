@@ -33,7 +33,7 @@ template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst)
 template <class T> void launch_flow_init(const Launch& g);   // F[0] <- U for every lattice
 template <class T> void launch_plaquette(const Launch& g, double* partial);
 template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
-template <class T> void launch_staples(const Launch& g, T* out_field);
+template <class T> void launch_staples(const Launch& g, double* out_aos);
 // sweep_base: optional device counter added to sweep_idx (graph replay);
 // grid: 0 = one block per tile, else a persistent grid (a multiple of 8)
 template <class T> void launch_hb_half(const Launch& g, int mu, int parity, int n_hits, uint32_t sweep_idx,

@@ -397,14 +397,20 @@ __global__ __launch_bounds__(kBlock) void k_finalize_qe(const LatDev<T>* __restr
 
 // -------------------------------------------------------------- staples ----
 
+// Staple sums are not SU(3), so they go out as 18 doubles per link in the
+// host's AoS order (lex site, mu), not through the link storage.
 template <class T, int MU>
-__device__ __forceinline__ void staple_one(const LatDev<T>& L, T* out, Coords c, int p, uint32_t i) {
-    st_link(out, L.S, MU, p, i, staple<T, MU>(L, L.U, c, p, i));
+__device__ __forceinline__ void staple_one(const LatDev<T>& L, double* aos, Coords c, int p, uint32_t i) {
+    const M3<T> a = staple<T, MU>(L, L.U, c, p, i);
+    const size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
+    double* d = aos + (lex * 4 + (size_t)MU) * 18;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) d[k] = (double)a.v[k];
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                    int n_lat, uint32_t nblocks, T* __restrict__ out) {
+                                                    int n_lat, uint32_t nblocks, double* __restrict__ aos) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
     const uint32_t q = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
@@ -414,10 +420,10 @@ __global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict_
     Coords c = tiled_coords(L, q, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     switch (mu) {
-        case 0: staple_one<T, 0>(L, out, c, p, i); break;
-        case 1: staple_one<T, 1>(L, out, c, p, i); break;
-        case 2: staple_one<T, 2>(L, out, c, p, i); break;
-        default: staple_one<T, 3>(L, out, c, p, i); break;
+        case 0: staple_one<T, 0>(L, aos, c, p, i); break;
+        case 1: staple_one<T, 1>(L, aos, c, p, i); break;
+        case 2: staple_one<T, 2>(L, aos, c, p, i); break;
+        default: staple_one<T, 3>(L, aos, c, p, i); break;
     }
 }
 
@@ -651,6 +657,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         uint32_t stash[2] = {0u, 0u};
         for (int h = 0; h < n_hits; ++h)
             ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh, stash);
+        su3_store_fix(U);
         st_link(L.U, L.S, MU, p, i, U);
         if (ex) atomicAdd(exhausted, (unsigned long long)ex);
     }
@@ -688,7 +695,9 @@ __device__ __forceinline__ void flow_link(const LatDev<T>& L, const T* __restric
         X = a9_axpby(T(0.75) * eps, Z, -eps, Zc);
     }
     M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
-    st_link(wout, L.S, MU, p, i, mul_nn(E, W));
+    M3<T> Wn = mul_nn(E, W);
+    su3_store_fix(Wn);
+    st_link(wout, L.S, MU, p, i, Wn);
 }
 
 template <class T, int STAGE>
@@ -820,7 +829,7 @@ void launch_clover(const Launch& g, int which, uint32_t gstep, double* pq, doubl
     hipLaunchKernelGGL(k_clover<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), which, gstep, pq, pe);
 }
 template <class T>
-void launch_staples(const Launch& g, T* out) {
+void launch_staples(const Launch& g, double* out) {
     hipLaunchKernelGGL(k_staples<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), out);
 }
 template <class T>
@@ -897,7 +906,7 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_flow_init<T>(const Launch&);                                                         \
     template void launch_plaquette<T>(const Launch&, double*);                                                \
     template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
-    template void launch_staples<T>(const Launch&, T*);                                                       \
+    template void launch_staples<T>(const Launch&, double*);                                                       \
     template void launch_reunit<T>(const Launch&);                                                            \
     template void launch_translate<T>(const Launch&, const int*, T*);                                         \
     template void launch_hb_half<T>(const Launch&, int, int, int, uint32_t, int, const double*, int,           \

@@ -140,9 +140,17 @@ struct alignas(2 * sizeof(T)) Cplx {
     T re, im;
 };
 
+// Complex entries stored per link.  FP64 links keep all nine (the oracle's
+// arithmetic, bit for bit); FP32 links keep rows 0 and 1 and rebuild row 2 on
+// load (su3_third_row), so every FP32 kernel moves 2/3 of the bytes.  An FP32
+// lattice therefore holds SU(3) matrices only, as a gauge field does.
+template <class T> struct LinkSlots { static constexpr uint32_t n = 9; };
+template <> struct LinkSlots<float> { static constexpr uint32_t n = 6; };
+
 // Complex-entry index of entry 0 of link (hs, i); entry k is 64*k further.
+template <class T>
 __device__ __forceinline__ size_t link_base(uint32_t S, int hs, uint32_t i) {
-    return ((size_t)hs * (S >> 6) + (i >> 6)) * 576u + (i & 63u);
+    return ((size_t)hs * (S >> 6) + (i >> 6)) * (64u * LinkSlots<T>::n) + (i & 63u);
 }
 
 // Field pointers live in descriptors (generic pointers); casting them to the
@@ -157,14 +165,15 @@ template <> struct Vec2<double> { typedef double type __attribute__((ext_vector_
 template <class T>
 __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, uint32_t i) {
     typedef typename Vec2<T>::type V2;
-    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + link_base(S, mu * 2 + p, i);
+    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + link_base<T>(S, mu * 2 + p, i);
     M3<T> m;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < (int)LinkSlots<T>::n; ++k) {
         V2 c = b[64 * k];
         m.v[2 * k] = c.x;
         m.v[2 * k + 1] = c.y;
     }
+    if (LinkSlots<T>::n == 6) su3_third_row(m);
     return m;
 }
 
@@ -174,9 +183,9 @@ __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, 
 template <class T>
 __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_t i, const M3<T>& m) {
     typedef typename Vec2<T>::type V2;
-    QCB_GLOBAL V2* b = (QCB_GLOBAL V2*)f + link_base(S, mu * 2 + p, i);
+    QCB_GLOBAL V2* b = (QCB_GLOBAL V2*)f + link_base<T>(S, mu * 2 + p, i);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
+    for (int k = 0; k < (int)LinkSlots<T>::n; ++k) {
         V2 v = {m.v[2 * k], m.v[2 * k + 1]};
         __builtin_nontemporal_store(v, b + 64 * k);
     }

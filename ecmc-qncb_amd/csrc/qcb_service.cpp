@@ -173,7 +173,7 @@ void devset_free(DevSet& d) {
     for (int f = 0; f < 3; ++f) (void)hipFree(d.d_blk[f]);
 }
 
-uint64_t req_bytes(const qcb_lattice& l) { return 4 * field_bytes(&l); }
+uint64_t req_bytes(const qcb_lattice& l) { return field_bytes(&l) + flow_bytes(&l); }
 
 }  // namespace
 
@@ -278,7 +278,7 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
             char* base = s->arena_base + off;
             const uint64_t fb = field_bytes(&r->lat);
             r->lat.U = base;
-            for (int k = 0; k < 3; ++k) r->lat.F[k] = base + (k + 1) * fb;
+            for (int k = 0; k < 3; ++k) r->lat.F[k] = base + (k + 1) * fb;   // F[2] holds force_bytes
             r->state = 1;
             r->fresh = true;
             r->t_admit = now_s(s);

@@ -32,6 +32,56 @@ __device__ __forceinline__ void m3_identity(M3<T>& a) {
     a.v[0] = a.v[8] = a.v[16] = T(1);
 }
 
+// Row 2 of an SU(3) matrix from rows 0 and 1: u2 = conj(u0 x u1), the row that
+// makes the rows orthonormal with determinant 1.
+template <class T>
+__device__ __forceinline__ void su3_third_row(M3<T>& m) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int a = (c + 1) % 3, b = (c + 2) % 3;
+        const T re = (m.re(0, a) * m.re(1, b) - m.im(0, a) * m.im(1, b)) -
+                     (m.re(0, b) * m.re(1, a) - m.im(0, b) * m.im(1, a));
+        const T im = (m.re(0, a) * m.im(1, b) + m.im(0, a) * m.re(1, b)) -
+                     (m.re(0, b) * m.im(1, a) + m.im(0, b) * m.re(1, a));
+        m.re(2, c) = re;
+        m.im(2, c) = -im;
+    }
+}
+
+// Before an updated FP32 link is stored (two-row storage), rows 0 and 1 are
+// made orthonormal (Gram-Schmidt; v_rsq_f32): row 2 is rebuilt from them at
+// every load, which would otherwise feed their norm errors back into the next
+// update and let them grow from sweep to sweep.  FP64 links store all nine
+// entries and are left exactly as computed.
+__device__ __forceinline__ void su3_store_fix(M3<double>&) {}
+__device__ __forceinline__ void su3_store_fix(M3<float>& m) {
+    float n0 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n0 = fmaf(m.re(0, c), m.re(0, c), fmaf(m.im(0, c), m.im(0, c), n0));
+    const float s0 = __builtin_amdgcn_rsqf(n0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { m.re(0, c) *= s0; m.im(0, c) *= s0; }
+    // r1 -= <r0, r1> r0 with <r0, r1> = sum conj(r0_c) r1_c
+    float dr = 0.f, di = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        dr = fmaf(m.re(0, c), m.re(1, c), fmaf(m.im(0, c), m.im(1, c), dr));
+        di = fmaf(m.re(0, c), m.im(1, c), fmaf(-m.im(0, c), m.re(1, c), di));
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float r = m.re(0, c), i = m.im(0, c);
+        m.re(1, c) -= dr * r - di * i;
+        m.im(1, c) -= dr * i + di * r;
+    }
+    float n1 = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n1 = fmaf(m.re(1, c), m.re(1, c), fmaf(m.im(1, c), m.im(1, c), n1));
+    const float s1 = __builtin_amdgcn_rsqf(n1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { m.re(1, c) *= s1; m.im(1, c) *= s1; }
+}
+
 // c = a * b
 template <class T>
 __device__ __forceinline__ M3<T> mul_nn(const M3<T>& a, const M3<T>& b) {

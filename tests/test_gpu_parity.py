@@ -10,6 +10,7 @@ import pytest
 
 import oracle_lib as qo
 import qcb
+from linkstore import assert_fp32_stored
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +47,7 @@ def test_roundtrip_and_cold(ctx):
     assert np.array_equal(g.get_links(), U)
     f = qcb.Lattice(ctx, dims, qcb.F32)
     f.set_links(U)
-    assert np.array_equal(f.get_links(), U.astype(np.complex64).astype(np.complex128))
+    assert_fp32_stored(f.get_links(), U)
 
 
 @pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 6)])
@@ -215,6 +216,24 @@ def test_heatbath_deterministic_and_su3(ctx):
     assert np.array_equal(a.get_links(), b.get_links())
     U = a.get_links().reshape(-1, 3, 3)
     assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-5
+
+
+def test_fp32_links_stay_su3_without_reunitarize(ctx):
+    """FP32 links keep rows 0 and 1 and rebuild row 2 at every load; the update
+    kernels re-orthonormalise the two rows before storing, so rounding errors
+    do not compound from sweep to sweep (without that, the rebuilt row feeds
+    the norm errors back and 300 sweeps overflow)."""
+    g = qcb.Lattice(ctx, (8, 8, 8, 8), qcb.F32)
+    g.hot(21)
+    g.hb_sweeps(6.0, 2, 300, seed=21)
+    U = g.get_links().reshape(-1, 3, 3)
+    assert np.isfinite(U).all()
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 2e-6
+    assert np.abs(np.linalg.det(U) - 1).max() < 2e-6
+    g.flow_copy()
+    g.flow_rk3(0.02, 150)
+    W = g.flow_links().reshape(-1, 3, 3)
+    assert np.abs(W @ W.conj().transpose(0, 2, 1) - np.eye(3)).max() < 2e-6
 
 
 def test_reunitarize_and_translate(ctx):
