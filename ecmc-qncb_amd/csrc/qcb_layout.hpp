@@ -177,6 +177,21 @@ __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, 
     return m;
 }
 
+// Rows 0 and 1 of an FP32 link as stored, row 2 left unset: for a product
+// whose row 2 is rebuilt afterwards (mul_nn / mul_nd with R = 2).
+__device__ __forceinline__ M3<float> ld_link_rows01(const float* f, uint32_t S, int mu, int p, uint32_t i) {
+    typedef Vec2<float>::type V2;
+    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + link_base<float>(S, mu * 2 + p, i);
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        V2 c = b[64 * k];
+        m.v[2 * k] = c.x;
+        m.v[2 * k + 1] = c.y;
+    }
+    return m;
+}
+
 // Every kernel writes whole fields that no thread of the same launch reads
 // back, so stores are non-temporal: they stream through L2 without evicting
 // the neighbour links the staples are still reusing.
@@ -265,19 +280,40 @@ __device__ __forceinline__ M3<T> staple(const LatDev<T>& L, const T* __restrict_
         const uint32_t xnu = nb_cb(L, c, nu, +1);
         const uint32_t xmnu = nb_cb(L, c, nu, -1);
         const uint32_t xmumnu = nb_cb(L, c, MU, +1, nu, -1);
-        {
-            M3<T> a = ld_link(f, S, nu, q, xmu);   // U_nu(x+mu)
-            M3<T> b = ld_link(f, S, MU, q, xnu);   // U_mu(x+nu)
-            M3<T> t = mul_nd(a, b);
-            M3<T> d = ld_link(f, S, nu, p, i);     // U_nu(x)
-            acc_nd(A, t, d);
-        }
-        {
-            M3<T> a = ld_link(f, S, MU, q, xmnu);    // U_mu(x-nu)
-            M3<T> b = ld_link(f, S, nu, p, xmumnu);  // U_nu(x+mu-nu)
-            M3<T> t = mul_nn(a, b);
-            M3<T> d = ld_link(f, S, nu, q, xmnu);    // U_nu(x-nu)
-            acc_dn(A, t, d);
+        if constexpr (LinkSlots<T>::n == 6) {
+            // FP32: the first factor's stored rows give rows 0, 1 of the
+            // product t (an SU(3) matrix), whose row 2 is then rebuilt
+            {
+                M3<T> a = ld_link_rows01(f, S, nu, q, xmu);   // U_nu(x+mu)
+                M3<T> b = ld_link(f, S, MU, q, xnu);          // U_mu(x+nu)
+                M3<T> t = mul_nd<T, 2>(a, b);
+                su3_third_row(t);
+                M3<T> d = ld_link(f, S, nu, p, i);            // U_nu(x)
+                acc_nd(A, t, d);
+            }
+            {
+                M3<T> a = ld_link_rows01(f, S, MU, q, xmnu);  // U_mu(x-nu)
+                M3<T> b = ld_link(f, S, nu, p, xmumnu);       // U_nu(x+mu-nu)
+                M3<T> t = mul_nn<T, 2>(a, b);
+                su3_third_row(t);
+                M3<T> d = ld_link(f, S, nu, q, xmnu);         // U_nu(x-nu)
+                acc_dn(A, t, d);
+            }
+        } else {
+            {
+                M3<T> a = ld_link(f, S, nu, q, xmu);   // U_nu(x+mu)
+                M3<T> b = ld_link(f, S, MU, q, xnu);   // U_mu(x+nu)
+                M3<T> t = mul_nd(a, b);
+                M3<T> d = ld_link(f, S, nu, p, i);     // U_nu(x)
+                acc_nd(A, t, d);
+            }
+            {
+                M3<T> a = ld_link(f, S, MU, q, xmnu);    // U_mu(x-nu)
+                M3<T> b = ld_link(f, S, nu, p, xmumnu);  // U_nu(x+mu-nu)
+                M3<T> t = mul_nn(a, b);
+                M3<T> d = ld_link(f, S, nu, q, xmnu);    // U_nu(x-nu)
+                acc_dn(A, t, d);
+            }
         }
     }
     return A;

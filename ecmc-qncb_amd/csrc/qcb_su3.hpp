@@ -39,12 +39,17 @@ __device__ __forceinline__ void su3_third_row(M3<T>& m) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         const int a = (c + 1) % 3, b = (c + 2) % 3;
-        const T re = (m.re(0, a) * m.re(1, b) - m.im(0, a) * m.im(1, b)) -
-                     (m.re(0, b) * m.re(1, a) - m.im(0, b) * m.im(1, a));
-        const T im = (m.re(0, a) * m.im(1, b) + m.im(0, a) * m.re(1, b)) -
-                     (m.re(0, b) * m.im(1, a) + m.im(0, b) * m.re(1, a));
+        // Re(u_a v_b - u_b v_a) and -Im(u_a v_b - u_b v_a), u = row 0, v = row 1
+        T re = m.re(0, a) * m.re(1, b);
+        re = fma(-m.im(0, a), m.im(1, b), re);
+        re = fma(-m.re(0, b), m.re(1, a), re);
+        re = fma(m.im(0, b), m.im(1, a), re);
+        T im = m.re(0, b) * m.im(1, a);
+        im = fma(m.im(0, b), m.re(1, a), im);
+        im = fma(-m.re(0, a), m.im(1, b), im);
+        im = fma(-m.im(0, a), m.re(1, b), im);
         m.re(2, c) = re;
-        m.im(2, c) = -im;
+        m.im(2, c) = im;
     }
 }
 
@@ -82,12 +87,12 @@ __device__ __forceinline__ void su3_store_fix(M3<float>& m) {
     for (int c = 0; c < 3; ++c) { m.re(1, c) *= s1; m.im(1, c) *= s1; }
 }
 
-// c = a * b
-template <class T>
+// c = a * b; with R = 2 only rows 0 and 1 of c are formed (and of a read)
+template <class T, int R = 3>
 __device__ __forceinline__ M3<T> mul_nn(const M3<T>& a, const M3<T>& b) {
     M3<T> c;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             T r = a.re(i, 0) * b.re(0, j);
@@ -107,12 +112,12 @@ __device__ __forceinline__ M3<T> mul_nn(const M3<T>& a, const M3<T>& b) {
     return c;
 }
 
-// c = a * b^dag
-template <class T>
+// c = a * b^dag; with R = 2 only rows 0 and 1 of c are formed (and of a read)
+template <class T, int R = 3>
 __device__ __forceinline__ M3<T> mul_nd(const M3<T>& a, const M3<T>& b) {
     M3<T> c;
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             // sum_k a(i,k) conj(b(j,k))
