@@ -334,8 +334,6 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         }
         n += 1;
         const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
-        const double gam[6] = {-log(u32d(o0.x)), -log(u32d(o0.y)), -log(u32d(o0.z)),
-                               -log(u32d(o0.w)), -log(u32d(o1.x)), -log(u32d(o1.y))};
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
         const M3<double> Ucur = ldl(L, a, st.site, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
@@ -362,7 +360,9 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
                 p11r += m1r * R.re(col, 1) - m1i * R.im(col, 1);
                 p11i += m1r * R.im(col, 1) + m1i * R.re(col, 1);
             }
-            const double gj = j == 0 ? gam[0] : j == 1 ? gam[1] : j == 2 ? gam[2] : j == 3 ? gam[3] : j == 4 ? gam[4] : gam[5];
+            // gamma_j = -log u_j: lane j takes only its own exponential draw
+            const uint32_t wj = j == 0 ? o0.x : j == 1 ? o0.y : j == 2 ? o0.z : j == 3 ? o0.w : j == 4 ? o1.x : o1.y;
+            const double gj = -log(u32d(wj));
             theta_reject = reject_fast((p00r + p11r) * beta_red, (p11i - p00i) * beta_red, gj, st.eps);
             // a NaN angle must win the reduction so the team stops
             if (!(theta_reject >= 0.0)) theta_reject = -1.0;
@@ -422,13 +422,20 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             plaq_links(a, st.site, st.mu, jmin, ls, ld);
             const M3<double> U1 = ldl(L, a, ls[1], ld[1]), U2 = ldl(L, a, ls[2], ld[2]), U3 = ldl(L, a, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
-            // lane i < 4 forms plaquette i; every lane then holds all four dS
-            double my_ds = 0.0;
+            // the proposed R' does not depend on the chosen plaquette, so it is
+            // formed first and lane i < 4 evaluates plaquette i against both R
+            // and R'; every lane then holds all four dS and dS'
+            const U4 o2 = ctr_ev(chain, n, 2, a);
+            const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
+            const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
+            double my_ds = 0.0, my_dsn = 0.0;
             if (lane < 4) {
                 const int i = lane;
                 const uint32_t si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
                 const int di = i == 0 ? ld[0] : i == 1 ? ld[1] : i == 2 ? ld[2] : ld[3];
-                my_ds = frozen(a, si, di) ? 0.0 : compute_ds(plaq_P(i, fwd, U0, U1, U2, U3), R);
+                const M3<double> P = plaq_P(i, fwd, U0, U1, U2, U3);
+                my_ds = frozen(a, si, di) ? 0.0 : compute_ds(P, R);
+                my_dsn = compute_ds(P, Rn);
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
             double absd[4], sum = 0.0;
@@ -445,12 +452,8 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             for (int i = 0; i < 4; ++i) prob[i] = absd[i] / sum;
             const double u = u32d(o1.z);   // selectVariable (ecmc_mpi_cb.cpp:252-260)
             const int idx = u < prob[0] ? 0 : (u < prob[0] + prob[1] ? 1 : (u < prob[0] + prob[1] + prob[2] ? 2 : 3));
-            const U4 o2 = ctr_ev(chain, n, 2, a);
-            const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
-            const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
-            const M3<double> Pidx = plaq_P(idx, fwd, U0, U1, U2, U3);
             const double dS = sgn[idx] * absd[idx];
-            const double dSn = compute_ds(Pidx, Rn);
+            const double dSn = __shfl(my_dsn, base + idx, 64);
             double neps = -sgn[idx];
             if (fabs(dS) < fabs(dSn) || u32d(o1.w) < fabs(dSn) / fabs(dS)) {
                 R = Rn;
