@@ -15,7 +15,10 @@
  *    host pointers are complete when the call returns.
  *  - prec: QCB_F64 (links and arithmetic in double) or QCB_F32 (links,
  *    staples and flow stages in float; global sums, flow time and
- *    reunitarisation in double).
+ *    reunitarisation in double).  An FP32 lattice stores rows 0 and 1 of
+ *    each link and rebuilds row 2 as conj(row 0 x row 1), so it holds SU(3)
+ *    matrices: qcb_lattice_set_links keeps rows 0 and 1 of its input and
+ *    qcb_lattice_get_links returns the rebuilt row 2.
  *  - Lattice extents are periodic, each >= 4, Lx even (even/odd split);
  *    L^3 x T lattices are allowed (the reference is hypercubic only).
  */
