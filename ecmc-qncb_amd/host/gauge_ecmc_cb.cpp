@@ -210,14 +210,65 @@ void generate_ecmc_cb(const RunParamsECB& rp, bool existing) {
     check(qcb_ecmc_destroy(ecmc), "qcb_ecmc_destroy");
 }
 
+// --print-states: the chain states a resume of this input would load, one
+// line per chain (device chain index, rank of the file, global site coords,
+// mu, epsilon, counters, Philox position); no device is touched.
+int print_states(const RunParamsECB& rp) {
+    const int n = rp.n_core_dims * rp.n_core_dims * rp.n_core_dims * rp.n_core_dims;
+    const int L = rp.L_core * rp.n_core_dims;
+    std::vector<ChainState> st(n);
+    try {
+        if (!qcb_host::io::load_states(st, rp.L_core, rp.n_core_dims, rp.run_name, rp.run_dir)) {
+            std::cerr << "No chain state files for run " << rp.run_name << " in " << rp.run_dir << "\n";
+            return 1;
+        }
+    } catch (const std::exception& ex) {
+        std::cerr << ex.what() << "\n";
+        return 1;
+    }
+    std::cout << std::setprecision(17);
+    for (int c = 0; c < n; ++c) {
+        const ChainState& s = st[c];
+        uint64_t x = s.site;
+        int co[4];
+        for (int d = 0; d < 4; ++d) {
+            co[d] = (int)(x % (uint64_t)L);
+            x /= (uint64_t)L;
+        }
+        std::cout << "chain " << c << " site " << co[0] << " " << co[1] << " " << co[2] << " " << co[3] << " mu "
+                  << s.mu << " eps " << s.eps << " par_rs " << s.par_refresh_site << " par_rR " << s.par_refresh_R
+                  << " th_rs " << s.th_refresh_site << " th_rR " << s.th_refresh_R << " set_counter " << s.set_counter
+                  << " events " << s.events << " lifts " << s.lifts << " revs " << s.revs << " init "
+                  << s.initialized << " rng_n " << s.rng_n << " set_gen " << s.set_gen << " R";
+        for (int k = 0; k < 18; ++k) std::cout << " " << s.R[k];
+        std::cout << "\n";
+    }
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char* argv[]) {
     if (argc < 2) {
-        std::cerr << "Usage: " << argv[0] << " <input_file.txt>" << std::endl;
+        std::cerr << "Usage: " << argv[0] << " [--print-params | --print-states] <input_file.txt>" << std::endl;
         return 1;
     }
     RunParamsECB params;
+    const std::string mode = argv[1];
+    if ((mode == "--print-params" || mode == "--print-states") && argc >= 3) {
+        // parse (and read the saved chain states) only: no device, no output files
+        try {
+            qcb_host::io::load_params(argv[2], params);
+        } catch (const std::exception& e) {
+            std::cerr << "Error reading input : " << e.what() << std::endl;
+            return 1;
+        }
+        if (mode == "--print-states") return print_states(params);
+        qcb_host::print_parameters(params);
+        std::cout << "N_samples=" << params.ecmc_params.N_samples << " N_switch_eo=" << params.N_switch_eo
+                  << " cold_start=" << (params.cold_start ? "true" : "false") << "\n";
+        return 0;
+    }
     bool existing = qcb_host::io::read_params(params, argv[1]);
     init_device(0);
     auto start = std::chrono::steady_clock::now();
