@@ -291,7 +291,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=10)
+    # ~0.1 s of heatbath sweeps: the GPU reaches its steady clocks under load
+    # only after that (10 warmup sweeps time 62 us per half-sweep, 200 time 56 us)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--flow-steps", type=int, default=10)
     ap.add_argument("--L", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
