@@ -74,6 +74,13 @@ def test_plaquette_clover_staples(ctx, dims):
     st = g.staples()
     want = np.stack([[o.staple(s, mu) for mu in range(4)] for s in range(g.V)])
     assert maxrel(st, want) < REL
+    # FP32 (two-row links, rows rebuilt inside the staple products): float
+    # rounding of six sums of two products
+    f = qcb.Lattice(ctx, dims, qcb.F32)
+    f.set_links(o.links)
+    assert np.abs(f.staples() - want).max() < 2e-5
+    assert abs(f.plaquette() - o.plaquette()) < 1e-6
+    f.close()
 
 
 def test_abelian_flux_closed_form(ctx):
