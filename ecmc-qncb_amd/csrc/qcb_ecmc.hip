@@ -67,34 +67,59 @@ __device__ inline uint32_t glx(const EcmcArgs& a, const int c[4]) {
     return (uint32_t)c[0] + (uint32_t)a.L[0] * ((uint32_t)c[1] + (uint32_t)a.L[1] * ((uint32_t)c[2] + (uint32_t)a.L[2] * (uint32_t)c[3]));
 }
 
-__device__ inline uint32_t gsh(const EcmcArgs& a, uint32_t s, int d1, int s1, int d2 = 0, int s2 = 0) {
+// Inside the event loop a site is carried as its coordinates: a neighbour is
+// a wrap of one coordinate and a link address a few small multiplies, so no
+// integer division is left in the loop (the lex index is formed only to
+// address memory and to keep the chain state).
+struct Site {
     int c[4];
-    gco(a, s, c);
-    if (s1) c[d1] = (c[d1] + s1 + a.L[d1]) % a.L[d1];
-    if (s2) c[d2] = (c[d2] + s2 + a.L[d2]) % a.L[d2];
-    return glx(a, c);
+};
+
+__device__ inline Site site_of(const EcmcArgs& a, uint32_t s) {
+    Site x;
+    gco(a, s, x.c);
+    return x;
 }
 
-__device__ inline bool frozen(const EcmcArgs& a, uint32_t s, int mu) {
-    int c[4];
-    gco(a, s, c);
+__device__ inline Site gsh(const EcmcArgs& a, Site x, int d1, int s1, int d2 = 0, int s2 = 0) {
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const int sh = (d == d1 ? s1 : 0) + (d == d2 ? s2 : 0);
+        if (sh) {
+            int v = x.c[d] + sh;
+            v = v < 0 ? v + a.L[d] : (v >= a.L[d] ? v - a.L[d] : v);
+            x.c[d] = v;
+        }
+    }
+    return x;
+}
+
+__device__ inline bool same_site(const Site& x, const Site& y) {
+    return x.c[0] == y.c[0] && x.c[1] == y.c[1] && x.c[2] == y.c[2] && x.c[3] == y.c[3];
+}
+
+// frozen (x_d mod b_d == 0 for some d != mu) for the sites an event touches:
+// they lie within [org, org + b] of the chain's block origin per direction
+// (modulo L), where x_d mod b_d == 0 means an offset of 0 or b
+__device__ inline bool frozen_near(const EcmcArgs& a, const int org[4], const Site& x, int mu) {
     bool f = false;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) f = f || (d != mu && c[d] % a.b[d] == 0);
+    for (int d = 0; d < 4; ++d) {
+        int r = x.c[d] - org[d];
+        r = r < 0 ? r + a.L[d] : r;
+        f = f || (d != mu && (r == 0 || r == a.b[d]));
+    }
     return f;
 }
 
-// links by lex site: cb index = lex / 2 (Lx even), parity = coordinate sum
-__device__ inline M3<double> ldl(const LatDev<double>& L, const EcmcArgs& a, uint32_t s, int mu) {
-    int c[4];
-    gco(a, s, c);
-    return ld_link(L.U, L.S, mu, (c[0] + c[1] + c[2] + c[3]) & 1, s >> 1);
+// links by site: cb index = lex / 2 (Lx even), parity = coordinate sum
+__device__ inline M3<double> ldl(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu) {
+    return ld_link(L.U, L.S, mu, (x.c[0] + x.c[1] + x.c[2] + x.c[3]) & 1, glx(a, x.c) >> 1);
 }
 
-__device__ inline void stl(const LatDev<double>& L, const EcmcArgs& a, uint32_t s, int mu, const M3<double>& m) {
-    int c[4];
-    gco(a, s, c);
-    double2* b = (double2*)L.U + link_base<double>(L.S, mu * 2 + ((c[0] + c[1] + c[2] + c[3]) & 1), s >> 1);
+__device__ inline void stl(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu, const M3<double>& m) {
+    double2* b = (double2*)L.U + link_base<double>(L.S, mu * 2 + ((x.c[0] + x.c[1] + x.c[2] + x.c[3]) & 1),
+                                                   glx(a, x.c) >> 1);
 #pragma unroll
     for (int k = 0; k < 9; ++k) b[64 * k] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
 }
@@ -228,19 +253,19 @@ __device__ inline int dir_of(int mu, int j) {
     return k + (k >= mu ? 1 : 0);
 }
 
-__device__ M3<double> staple_j(const LatDev<double>& L, const EcmcArgs& a, uint32_t x, int mu, int j) {
+__device__ M3<double> staple_j(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu, int j) {
     const int nu = dir_of(mu, j);
     if ((j & 1) == 0) {
         M3<double> t = mul_nn(ldl(L, a, x, nu), ldl(L, a, gsh(a, x, nu, 1), mu));
         return mul_nd(ldl(L, a, gsh(a, x, mu, 1), nu), t);
     }
-    const uint32_t xmnu = gsh(a, x, nu, -1);
+    const Site xmnu = gsh(a, x, nu, -1);
     M3<double> t = mul_nn(ldl(L, a, xmnu, mu), ldl(L, a, gsh(a, x, mu, 1, nu, -1), nu));
     return mul_dn(t, ldl(L, a, xmnu, nu));
 }
 
 // the four links of plaquette j, the current link first (GeometryCB.cpp:60-90)
-__device__ inline void plaq_links(const EcmcArgs& a, uint32_t x, int mu, int j, uint32_t s[4], int d[4]) {
+__device__ inline void plaq_links(const EcmcArgs& a, const Site& x, int mu, int j, Site s[4], int d[4]) {
     const int nu = dir_of(mu, j);
     s[0] = x;
     d[0] = mu;
@@ -292,6 +317,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
     M3<double> R;
 #pragma unroll
     for (int k = 0; k < 18; ++k) R.v[k] = st.R[k];
+    Site xs = site_of(a, st.site);   // the chain's current site
     auto draw_site = [&](unsigned long long n) {
         const U4 o3 = ctr_ev(chain, n, 3, a), o4 = ctr_ev(chain, n, 4, a);
         const uint32_t w[4] = {o3.x, o3.y, o3.z, o3.w};
@@ -299,6 +325,8 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
 #pragma unroll
         for (int d = 0; d < 4; ++d) c[d] = org[d] + 1 + (int)__umulhi(w[d], (uint32_t)(a.b[d] - 2));
         st.site = glx(a, c);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) xs.c[d] = c[d];
         st.mu = (int)(o4.x >> 30);
         st.eps = 2 * (int)(o4.y >> 31) - 1;
         return o4;
@@ -335,7 +363,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         n += 1;
         const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
-        const M3<double> Ucur = ldl(L, a, st.site, st.mu);
+        const M3<double> Ucur = ldl(L, a, xs, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
         // lane j < 6 solves staple j; the team takes the smallest angle, the
         // lowest j on ties (the sequential scan's first minimum)
@@ -343,7 +371,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         int jmin = lane;
         if (lane < 6) {
             const int j = lane;
-            const M3<double> S = staple_j(L, a, st.site, st.mu, j);
+            const M3<double> S = staple_j(L, a, xs, st.mu, j);
             double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
 #pragma unroll
             for (int col = 0; col < 3; ++col) {
@@ -388,13 +416,13 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         step = d_rs < step ? d_rs : step;
         step = d_rR < step ? d_rR : step;
         if (step == d_sample) {   // SAMPLE: the chain stops here until the next call
-            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_sample, st.eps, R));
+            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_sample, st.eps, R));
             st.events++;
             st.par_rs += d_sample;
             st.par_rR += d_sample;
             break;
         } else if (step == d_rs) {   // REFRESH SITE
-            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_rs, st.eps, R));
+            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_rs, st.eps, R));
             st.events++;
             par_sample += d_rs;
             st.par_rR += d_rs;
@@ -402,7 +430,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             const U4 o4 = draw_site(n);
             if (a.poisson) st.th_rs = -a.theta_refresh_site * log(u53(o4.z, o4.w));
         } else if (step == d_rR) {   // REFRESH R
-            stl(L, a, st.site, st.mu, ecmc_update(Ucur, d_rR, st.eps, R));
+            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_rR, st.eps, R));
             st.events++;
             par_sample += d_rR;
             st.par_rs += d_rR;
@@ -412,14 +440,14 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             R = haar_su3_philox(a.key0, a.key1, chain, (uint32_t)n, 0xED000000u | ((uint32_t)(n >> 32) & 0x00FFFFFFu));
         } else {   // LIFT (lift_improved_fast, ecmc_mpi_cb.cpp:277-354)
             const M3<double> U0 = ecmc_update(Ucur, theta_reject, st.eps, R);
-            stl(L, a, st.site, st.mu, U0);
+            stl(L, a, xs, st.mu, U0);
             st.events++;
             par_sample += theta_reject;
             st.par_rs += theta_reject;
             st.par_rR += theta_reject;
-            uint32_t ls[4];
+            Site ls[4];
             int ld[4];
-            plaq_links(a, st.site, st.mu, jmin, ls, ld);
+            plaq_links(a, xs, st.mu, jmin, ls, ld);
             const M3<double> U1 = ldl(L, a, ls[1], ld[1]), U2 = ldl(L, a, ls[2], ld[2]), U3 = ldl(L, a, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
             // the proposed R' does not depend on the chosen plaquette, so it is
@@ -431,10 +459,10 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             double my_ds = 0.0, my_dsn = 0.0;
             if (lane < 4) {
                 const int i = lane;
-                const uint32_t si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
+                const Site si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
                 const int di = i == 0 ? ld[0] : i == 1 ? ld[1] : i == 2 ? ld[2] : ld[3];
                 const M3<double> P = plaq_P(i, fwd, U0, U1, U2, U3);
-                my_ds = frozen(a, si, di) ? 0.0 : compute_ds(P, R);
+                my_ds = frozen_near(a, org, si, di) ? 0.0 : compute_ds(P, R);
                 my_dsn = compute_ds(P, Rn);
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
@@ -459,10 +487,12 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
                 R = Rn;
                 neps = -dsgn(dSn);
             }
-            st.revs += (ls[idx] == st.site) ? 1 : 0;
+            const Site xn = idx == 0 ? ls[0] : idx == 1 ? ls[1] : idx == 2 ? ls[2] : ls[3];
+            st.revs += same_site(xn, xs) ? 1 : 0;
             st.set_counter++;
             st.lifts++;
-            st.site = ls[idx];
+            xs = xn;
+            st.site = glx(a, xs.c);
             st.mu = ld[idx];
             st.eps = (int)neps;
         }
