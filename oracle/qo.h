@@ -2,7 +2,8 @@
  * qo.h -- FP64 CPU restatement of the reference's hot path (the oracle).
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the libqcb HIP library,
- * the gauge_hb_cb driver, the Python bindings) may link or call this code;
+ * the gauge_hb_cb and gauge_ecmc_cb drivers, the Python bindings) may link or
+ * call this code;
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
  * and only as the checker.
  *
@@ -134,11 +135,18 @@ typedef struct {             /* LocalChainState (ecmc_mpi_cb.h:15-35) plus the P
     size_t site;             /* global lex site */
     int mu, eps, initialized;
 } qo_ecmc_chain;
-double qo_ecmc_reject_fast(double A, double B, double gamma, int epsilon);   /* ecmc_mpi_cb.h:56-94 */
-double qo_ecmc_reject_exact(double A, double B, double gamma, int epsilon);  /* ecmc_mpi_cb.cpp:82-178 */
-double qo_ecmc_compute_ds(const double* Pi, const double* R);                /* ecmc_mpi_cb.cpp:263-275 */
-void qo_ecmc_update(double* U, double theta, int epsilon, const double* R);  /* ecmc_mpi_cb.cpp:357-372 */
-void qo_ecmc_random_su3_eps(double eps, const double r[9], double* M);       /* utils.cpp:66-103 */
+/* ecmc_mpi_cb.h:56-94 and ecmc_mpi_cb.cpp:82-178; both pinned by the defining
+ * integral (quadrature of the positive part of dS/dtheta between its zeros
+ * equals gamma) and by their mutual agreement */
+double qo_ecmc_reject_fast(double A, double B, double gamma, int epsilon);
+double qo_ecmc_reject_exact(double A, double B, double gamma, int epsilon);
+/* ecmc_mpi_cb.cpp:263-275 and :357-372; pinned by numpy matrix expressions
+ * Im[(R^dag P R)_00 - (R^dag P R)_11] and R diag(e^{i xi}, e^{-i xi}, 1) R^dag U */
+double qo_ecmc_compute_ds(const double* Pi, const double* R);
+void qo_ecmc_update(double* U, double theta, int epsilon, const double* R);
+/* utils.cpp:66-103; pinned by the embedded-SU(2) product form and unitarity */
+void qo_ecmc_random_su3_eps(double eps, const double r[9], double* M);
+/* one chain per block; pinned by every chain's site lying in its own block */
 int qo_ecmc_n_chains(const qo_lat* l, const qo_ecmc_params* p);
 /* One sample (sample_persistant, ecmc_mpi_cb.cpp:496-637) of every chain of
  * the block decomposition with the device's Philox counters:
@@ -154,8 +162,15 @@ int qo_ecmc_n_chains(const qo_lat* l, const qo_ecmc_params* p);
  *   matrix set, generation g (+1 per sample call and per 10000 lifts):
  *          element 2m+1 from {chain, g, 0xE5000000 | m, 0..2} (9 words u32 - 1/2),
  *          element 2m+2 its adjoint, element 0 the identity. */
+/* Pinned by: frozen links untouched, SU(3) kept, chains of different blocks
+ * independent (a perturbation in one block changes no other block), and the
+ * device chains' equilibrium plaquette against the device heatbath.  The
+ * event sequence itself (which event comes when) has no check independent of
+ * this restatement: parity unpinned beyond those invariants. */
 int qo_ecmc_sample_philox(qo_lat* l, const qo_ecmc_params* p, qo_ecmc_chain* chains, uint64_t seed);
-/* The reference's one-rank chain (the whole lattice one block), mt19937_64. */
+/* The reference's one-rank chain (the whole lattice one block), mt19937_64;
+ * pinned by the same invariants (the draw order of the reference's std
+ * distributions is restated from its source: parity unpinned beyond them). */
 int qo_ecmc_sample_ref(qo_lat* l, const qo_ecmc_params* p, qo_ecmc_chain* chain, qo_mt64* g);
 
 double qo_now(void);
