@@ -271,6 +271,21 @@ def precision_report(qcb, ctx, L=32, sweeps=500, flow_steps=150, seed=11, beta=6
                 equilibrium=equilibrium, plaquette_measurement=p_meas, flow=rows)
 
 
+def wf_cpu_baseline(L=16, steps=2):
+    """The reference's RK3 Wilson flow step (FP64, src/flow/gradient_flow.cpp:15-128)
+    restated by the oracle, one core, on a hot L^4 lattice: site-steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as qo
+    lat = qo.Lattice((L,) * 4)
+    lat.hot_philox(11)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lat.flow_step(0.02)
+    dt = time.perf_counter() - t0
+    return dict(value=steps * L ** 4 / dt, unit="site-steps/s", cores=1, kind="port",
+                sample=f"{steps} RK3 steps (eps = 0.02, FP64) of a hot {L}^4 lattice in {dt:.1f} s")
+
+
 def cpu_baseline(L=32):
     """One reference-order FP64 sweep (N_hits = 2, beta = 6.0) of a hot L^4
     lattice on one core; counts only the non-frozen links it updates."""
@@ -365,6 +380,8 @@ def main():
         if not args.no_ecmc:
             comp["ecmc_cpu_baseline"] = ecmc_cpu_baseline()
             log(f"[bench] CPU ECMC {comp['ecmc_cpu_baseline']['value']:.3e} events/s")
+        comp["wf_cpu_baseline"] = wf_cpu_baseline()
+        log(f"[bench] CPU WF {comp['wf_cpu_baseline']['value']:.3e} site-steps/s")
 
     V = L ** 4
     out = {
