@@ -675,51 +675,141 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 //  stage 2: Zs = 8/9 Z - 17/36 Zs,       W' = exp(eps Zs) W
 //  stage 3:                              W' = exp(eps (3/4 Z - Zs)) W
 // Z and Zs are carried as their 9 independent reals (A9).
-template <class T, int MU, int STAGE>
-__device__ __forceinline__ void flow_link(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
-                                          T* __restrict__ zs, Coords c, int p, uint32_t i, T eps) {
-    M3<T> Sg = staple<T, MU>(L, win, c, p, i);
-    M3<T> W = ld_link(win, L.S, MU, p, i);
+template <class T, int STAGE>
+__device__ __forceinline__ void flow_tail(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
+                                          T* __restrict__ zs, int mu, int p, uint32_t i, T eps, const M3<T>& Sg) {
+    M3<T> W = ld_link(win, L.S, mu, p, i);
     A9<T> Z = force_ta(W, Sg);
     A9<T> X;
     if (STAGE == 1) {
-        st_a9(zs, L.S, MU, p, i, Z);
+        st_a9(zs, L.S, mu, p, i, Z);
         X = a9_scale(T(0.25) * eps, Z);
     } else if (STAGE == 2) {
-        A9<T> Z0 = ld_a9((const T*)zs, L.S, MU, p, i);
+        A9<T> Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
         A9<T> Z1 = a9_axpby(T(8.0 / 9.0), Z, T(-17.0 / 36.0), Z0);
-        st_a9(zs, L.S, MU, p, i, Z1);
+        st_a9(zs, L.S, mu, p, i, Z1);
         X = a9_scale(eps, Z1);
     } else {
-        A9<T> Zc = ld_a9((const T*)zs, L.S, MU, p, i);
+        A9<T> Zc = ld_a9((const T*)zs, L.S, mu, p, i);
         X = a9_axpby(T(0.75) * eps, Z, -eps, Zc);
     }
     M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
     M3<T> Wn = mul_nn(E, W);
     su3_store_fix(Wn);
-    st_link(wout, L.S, MU, p, i, Wn);
+    st_link(wout, L.S, mu, p, i, Wn);
+}
+
+// The block's 256 threads are 32 sites x 8 (mu, parity) classes, so the four
+// links of a site sit in four wavefronts of one block.  In plane {a, b}
+// (a < b) the product S = U_b(x+a) U_a(x+b)^dag is the start of both upper
+// staples through x,
+//   Sigma_a += S U_b(x)^dag,   Sigma_b += S^dag U_a(x)^dag,
+// so one of the two threads forms it (owners spread 2/1/2/1 over the four
+// wavefronts) and hands it to the other through LDS; the lower staples
+// U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b) are not shared.  Phase 1: owned S into
+// LDS, then the three lower staples; barrier; phase 2: the three upper
+// staples and the update.
+__device__ __forceinline__ constexpr int plane_idx(int a, int b) { return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5); }
+__device__ __forceinline__ constexpr int plane_owner(int pl) {
+    return pl == 0 ? 0 : pl == 1 ? 2 : pl == 2 ? 0 : pl == 3 ? 1 : pl == 4 ? 3 : 2;
+}
+// LDS slot of entry k of plane pl's S at parity p, site s of the block
+__device__ __forceinline__ constexpr uint32_t plane_slot(int pl, int p, int k, uint32_t s) {
+    return ((uint32_t)(pl * 2 + p) * 18u + (uint32_t)k) * kLinkSites + s;
+}
+
+template <class T, int MU>
+__device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
+                                            uint32_t s, T* lds, M3<T>& Sg) {
+    m3_zero(Sg);
+    const uint32_t S = L.S;
+    const int q = p ^ 1;
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == MU) continue;
+        const int a = MU < nu ? MU : nu, b = MU < nu ? nu : MU;
+        const int pl = plane_idx(a, b);
+        if (plane_owner(pl) == MU) {
+            const uint32_t xa = nb_cb(L, c, a, +1), xb = nb_cb(L, c, b, +1);
+            M3<T> t;
+            if constexpr (LinkSlots<T>::n == 6) {
+                M3<T> u = ld_link_rows01(win, S, b, q, xa);
+                t = mul_nd<T, 2>(u, ld_link(win, S, a, q, xb));
+                su3_third_row(t);
+            } else {
+                t = mul_nd(ld_link(win, S, b, q, xa), ld_link(win, S, a, q, xb));
+            }
+#pragma unroll
+            for (int k = 0; k < 18; ++k) lds[plane_slot(pl, p, k, s)] = t.v[k];
+        }
+        const uint32_t xmnu = nb_cb(L, c, nu, -1);
+        const uint32_t xmumnu = nb_cb(L, c, MU, +1, nu, -1);
+        M3<T> t;
+        if constexpr (LinkSlots<T>::n == 6) {
+            M3<T> u = ld_link_rows01(win, S, MU, q, xmnu);   // U_mu(x-nu)
+            t = mul_nn<T, 2>(u, ld_link(win, S, nu, p, xmumnu));
+            su3_third_row(t);
+        } else {
+            t = mul_nn(ld_link(win, S, MU, q, xmnu), ld_link(win, S, nu, p, xmumnu));
+        }
+        acc_dn(Sg, t, ld_link(win, S, nu, q, xmnu));           // .. U_nu(x-nu)
+    }
+}
+
+template <class T, int MU, int STAGE>
+__device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
+                                            T* __restrict__ zs, int p, uint32_t i, uint32_t s, const T* lds,
+                                            M3<T>& Sg, T eps) {
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) {
+        if (nu == MU) continue;
+        const int pl = plane_idx(MU < nu ? MU : nu, MU < nu ? nu : MU);
+        M3<T> t;
+#pragma unroll
+        for (int k = 0; k < 18; ++k) t.v[k] = lds[plane_slot(pl, p, k, s)];
+        const M3<T> d = ld_link(win, L.S, nu, p, i);   // U_nu(x)
+        if (MU < nu) acc_nd(Sg, t, d);
+        else acc_dd(Sg, t, d);
+    }
+    flow_tail<T, STAGE>(L, win, wout, zs, MU, p, i, eps, Sg);
 }
 
 template <class T, int STAGE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::flow))) void k_flow_stage(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, uint32_t gstep, T eps) {
+    __shared__ T lds[6 * 2 * 18 * kLinkSites];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    const uint32_t q = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
+    const uint32_t s = threadIdx.x & (kLinkSites - 1);
+    const uint32_t q = bc.local * kLinkSites + s;
     const uint32_t hs = threadIdx.x / kLinkSites;
-    if (q >= L.Vh) return;
-    int mu = (int)(hs >> 1), p = (int)(hs & 1);
+    // every thread reaches the barrier; a site past the end has all four of
+    // its links past the end, so no live thread reads what it did not write
+    const bool live = q < L.Vh;
+    const int mu = (int)(hs >> 1), p = (int)(hs & 1);
     const int cur = (int)((gstep - L.flow_start) & 1u);
     const T* win = L.F[(STAGE == 2) ? 1 - cur : cur];
     T* wout = L.F[(STAGE == 2) ? cur : 1 - cur];
     T* zs = L.F[2];
-    Coords c = tiled_coords(L, q, p);
+    Coords c = tiled_coords(L, live ? q : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-    switch (mu) {
-        case 0: flow_link<T, 0, STAGE>(L, win, wout, zs, c, p, i, eps); break;
-        case 1: flow_link<T, 1, STAGE>(L, win, wout, zs, c, p, i, eps); break;
-        case 2: flow_link<T, 2, STAGE>(L, win, wout, zs, c, p, i, eps); break;
-        default: flow_link<T, 3, STAGE>(L, win, wout, zs, c, p, i, eps); break;
+    M3<T> Sg;
+    if (live) {
+        switch (mu) {
+            case 0: flow_phase1<T, 0>(L, win, c, p, s, lds, Sg); break;
+            case 1: flow_phase1<T, 1>(L, win, c, p, s, lds, Sg); break;
+            case 2: flow_phase1<T, 2>(L, win, c, p, s, lds, Sg); break;
+            default: flow_phase1<T, 3>(L, win, c, p, s, lds, Sg); break;
+        }
+    }
+    __syncthreads();
+    if (live) {
+        switch (mu) {
+            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
+            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
+            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
+            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
+        }
     }
 }
 

@@ -204,6 +204,26 @@ __device__ __forceinline__ void acc_dn(M3<T>& acc, const M3<T>& a, const M3<T>& 
         }
 }
 
+// acc += a^dag * b^dag = conj(sum_k a(k,i) b(j,k))
+template <class T>
+__device__ __forceinline__ void acc_dd(M3<T>& acc, const M3<T>& a, const M3<T>& b) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            T r = acc.re(i, j), s = acc.im(i, j);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                r = fma(a.re(k, i), b.re(j, k), r);
+                r = fma(-a.im(k, i), b.im(j, k), r);
+                s = fma(-a.re(k, i), b.im(j, k), s);
+                s = fma(-a.im(k, i), b.re(j, k), s);
+            }
+            acc.re(i, j) = r;
+            acc.im(i, j) = s;
+        }
+}
+
 template <class T>
 __device__ __forceinline__ T retrace(const M3<T>& a) {
     return a.v[0] + a.v[8] + a.v[16];
