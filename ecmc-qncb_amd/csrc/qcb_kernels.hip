@@ -53,6 +53,24 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
     return r;   // valid on thread 0
 }
 
+// Occupancy floor of the heavy kernels: with no floor the scheduler hoists
+// every neighbour load and the double kernels fall to one wave per SIMD.
+#ifndef QCB_HB_WAVES_F32
+#define QCB_HB_WAVES_F32 3
+#endif
+#ifndef QCB_FLOW_WAVES_F32
+#define QCB_FLOW_WAVES_F32 4
+#endif
+#ifndef QCB_HB_WAVES_F64
+#define QCB_HB_WAVES_F64 2
+#endif
+#ifndef QCB_FLOW_WAVES_F64
+#define QCB_FLOW_WAVES_F64 2
+#endif
+template <class T> struct MinWaves;
+template <> struct MinWaves<float> { static constexpr int hb = QCB_HB_WAVES_F32, flow = QCB_FLOW_WAVES_F32, clover = 3; };
+template <> struct MinWaves<double> { static constexpr int hb = QCB_HB_WAVES_F64, flow = QCB_FLOW_WAVES_F64, clover = 1; };
+
 // ---------------------------------------------------------------- fills ----
 
 template <class T>
@@ -268,8 +286,9 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
 
 // F_{mu nu}(x) = (C - C^dag)/8 of the four-leaf clover, not made traceless
 // (reference src/observables/observables_mpi.cpp:61-85).
-template <class T, int MU, int NU>
-__device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i) {
+template <class T>
+__device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i,
+                                          const int MU, const int NU) {
     const uint32_t S = L.S;
     const int q = p ^ 1;
     const uint32_t xpmu = nb_cb(L, c, MU, +1), xpnu = nb_cb(L, c, NU, +1);
@@ -283,16 +302,19 @@ __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restric
         M3<T> b = mul_nn(ld_link(f, S, NU, p, i), ld_link(f, S, MU, q, xpnu));
         C = mul_nd(a, b);
     }
+    __builtin_amdgcn_sched_barrier(0);   // one leaf's loads at a time: keeps the registers down
     {   // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
         M3<T> a = mul_nd(ld_link(f, S, NU, p, i), ld_link(f, S, MU, p, xmmupnu));
         M3<T> b = mul_dn(ld_link(f, S, MU, q, xmmu), ld_link(f, S, NU, q, xmmu));
         acc_nd(C, a, b);
     }
+    __builtin_amdgcn_sched_barrier(0);
     {   // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
         M3<T> a = mul_nn(ld_link(f, S, NU, p, xmmumnu), ld_link(f, S, MU, q, xmmu));
         M3<T> b = mul_nn(ld_link(f, S, MU, p, xmmumnu), ld_link(f, S, NU, q, xmnu));
         acc_dn(C, a, b);
     }
+    __builtin_amdgcn_sched_barrier(0);
     {   // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
         M3<T> a = mul_nn(ld_link(f, S, MU, q, xmnu), ld_link(f, S, NU, p, xpmumnu));
         M3<T> b = mul_dn(ld_link(f, S, NU, q, xmnu), a);
@@ -321,7 +343,7 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 // q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
 // (reference src/observables/observables_mpi.cpp:88-107).
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::clover))) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                    int n_lat, uint32_t nblocks, int which, uint32_t gstep,
                                                    double* __restrict__ pq, double* __restrict__ pe) {
     __shared__ double lds[kBlock / 64];
@@ -335,22 +357,16 @@ __global__ __launch_bounds__(kBlock) void k_clover(const LatDev<T>* __restrict__
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
         const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
         double tq = 0.0;
-        {
-            M3<T> a = clover_F<T, 0, 1>(L, f, c, p, i);
-            M3<T> b = clover_F<T, 2, 3>(L, f, c, p, i);
-            tq += (double)retrace_nn(a, b);
-            e += sqnorm(a) + sqnorm(b);
-        }
-        {
-            M3<T> a = clover_F<T, 0, 2>(L, f, c, p, i);
-            M3<T> b = clover_F<T, 1, 3>(L, f, c, p, i);
-            tq -= (double)retrace_nn(a, b);
-            e += sqnorm(a) + sqnorm(b);
-        }
-        {
-            M3<T> a = clover_F<T, 0, 3>(L, f, c, p, i);
-            M3<T> b = clover_F<T, 1, 2>(L, f, c, p, i);
-            tq += (double)retrace_nn(a, b);
+        // the three plane pairs (01,23), (02,13), (03,12) one at a time: a
+        // loop the compiler keeps rolled bounds the registers to one pair
+#pragma unroll 1
+        for (int pr = 0; pr < 3; ++pr) {
+            const int nu = pr + 1;                            // (0, nu) and its complement
+            const int m2 = pr == 2 ? 1 : (pr == 1 ? 1 : 2), n2 = pr == 0 ? 3 : (pr == 1 ? 3 : 2);
+            M3<T> a = clover_F<T>(L, f, c, p, i, 0, nu);
+            M3<T> b = clover_F<T>(L, f, c, p, i, m2, n2);
+            const double tr = (double)retrace_nn(a, b);
+            tq += pr == 1 ? -tr : tr;
             e += sqnorm(a) + sqnorm(b);
         }
         q = tq * (1.0 / (4.0 * M_PI * M_PI));
@@ -612,23 +628,6 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
 
 // One (mu, parity) half-sweep: each thread owns one link U_mu(x), computes its
 // staple once and applies n_hits CM hits in registers.
-// Occupancy floor of the heavy kernels: with no floor the scheduler hoists
-// every neighbour load and the double kernels fall to one wave per SIMD.
-#ifndef QCB_HB_WAVES_F32
-#define QCB_HB_WAVES_F32 3
-#endif
-#ifndef QCB_FLOW_WAVES_F32
-#define QCB_FLOW_WAVES_F32 4
-#endif
-#ifndef QCB_HB_WAVES_F64
-#define QCB_HB_WAVES_F64 2
-#endif
-#ifndef QCB_FLOW_WAVES_F64
-#define QCB_FLOW_WAVES_F64 2
-#endif
-template <class T> struct MinWaves;
-template <> struct MinWaves<float> { static constexpr int hb = QCB_HB_WAVES_F32, flow = QCB_FLOW_WAVES_F32; };
-template <> struct MinWaves<double> { static constexpr int hb = QCB_HB_WAVES_F64, flow = QCB_FLOW_WAVES_F64; };
 
 template <class T, int MU, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::hb))) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
