@@ -286,6 +286,23 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
 
 // F_{mu nu}(x) = (C - C^dag)/8 of the four-leaf clover, not made traceless
 // (reference src/observables/observables_mpi.cpp:61-85).
+// a b (or a b^dag) with the SU(3) factor a read from memory: an FP32 link
+// gives its two stored rows, rows 0 and 1 of the product follow and row 2 is
+// rebuilt; FP64 forms the full product
+template <class T, bool DAG>
+__device__ __forceinline__ M3<T> mul_ld(const T* __restrict__ f, uint32_t S, int mu, int p, uint32_t i,
+                                        const M3<T>& b) {
+    if constexpr (LinkSlots<T>::n == 6) {
+        const M3<T> a = ld_link_rows01(f, S, mu, p, i);
+        M3<T> t = DAG ? mul_nd<T, 2>(a, b) : mul_nn<T, 2>(a, b);
+        su3_third_row(t);
+        return t;
+    } else {
+        const M3<T> a = ld_link(f, S, mu, p, i);
+        return DAG ? mul_nd(a, b) : mul_nn(a, b);
+    }
+}
+
 template <class T>
 __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i,
                                           const int MU, const int NU) {
@@ -297,26 +314,25 @@ __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restric
     const uint32_t xpmumnu = nb_cb(L, c, MU, +1, NU, -1);
     M3<T> C;
     {   // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        M3<T> umu = ld_link(f, S, MU, p, i);
-        M3<T> a = mul_nn(umu, ld_link(f, S, NU, q, xpmu));
-        M3<T> b = mul_nn(ld_link(f, S, NU, p, i), ld_link(f, S, MU, q, xpnu));
+        M3<T> a = mul_ld<T, false>(f, S, MU, p, i, ld_link(f, S, NU, q, xpmu));
+        M3<T> b = mul_ld<T, false>(f, S, NU, p, i, ld_link(f, S, MU, q, xpnu));
         C = mul_nd(a, b);
     }
     __builtin_amdgcn_sched_barrier(0);   // one leaf's loads at a time: keeps the registers down
     {   // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
-        M3<T> a = mul_nd(ld_link(f, S, NU, p, i), ld_link(f, S, MU, p, xmmupnu));
+        M3<T> a = mul_ld<T, true>(f, S, NU, p, i, ld_link(f, S, MU, p, xmmupnu));
         M3<T> b = mul_dn(ld_link(f, S, MU, q, xmmu), ld_link(f, S, NU, q, xmmu));
         acc_nd(C, a, b);
     }
     __builtin_amdgcn_sched_barrier(0);
     {   // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        M3<T> a = mul_nn(ld_link(f, S, NU, p, xmmumnu), ld_link(f, S, MU, q, xmmu));
-        M3<T> b = mul_nn(ld_link(f, S, MU, p, xmmumnu), ld_link(f, S, NU, q, xmnu));
+        M3<T> a = mul_ld<T, false>(f, S, NU, p, xmmumnu, ld_link(f, S, MU, q, xmmu));
+        M3<T> b = mul_ld<T, false>(f, S, MU, p, xmmumnu, ld_link(f, S, NU, q, xmnu));
         acc_dn(C, a, b);
     }
     __builtin_amdgcn_sched_barrier(0);
     {   // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
-        M3<T> a = mul_nn(ld_link(f, S, MU, q, xmnu), ld_link(f, S, NU, p, xpmumnu));
+        M3<T> a = mul_ld<T, false>(f, S, MU, q, xmnu, ld_link(f, S, NU, p, xpmumnu));
         M3<T> b = mul_dn(ld_link(f, S, NU, q, xmnu), a);
         acc_nd(C, b, ld_link(f, S, MU, p, i));
         // C now holds all four leaves (acc_nd adds b * U_mu(x)^dag)
