@@ -468,8 +468,10 @@ __global__ __launch_bounds__(kBlock) void k_staples(const LatDev<T>* __restrict_
 // sin2pi(u) take the angle in turns, which is what v_cos_f32 / v_sin_f32 take.
 struct KpMath {
     __device__ static __forceinline__ double ln(double x) { return log(x); }
-    __device__ static __forceinline__ double cos2pi(double u) { return cos(2.0 * M_PI * u); }
-    __device__ static __forceinline__ void sincos2pi(double u, double* s, double* c) { sincos(2.0 * M_PI * u, s, c); }
+    // cos(2 pi u) as cospi(2u): the argument stays exact and the reduction is
+    // a quarter-turn split, not the large-argument path of cos
+    __device__ static __forceinline__ double cos2pi(double u) { return cospi(2.0 * u); }
+    __device__ static __forceinline__ void sincos2pi(double u, double* s, double* c) { sincospi(2.0 * u, s, c); }
     __device__ static __forceinline__ double sq(double x) { return sqrt(x); }
     __device__ static __forceinline__ float ln(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
     __device__ static __forceinline__ float cos2pi(float u) { return __builtin_amdgcn_cosf(u); }
