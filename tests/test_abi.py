@@ -35,3 +35,15 @@ def test_context_creation_reports_missing_gpu():
         assert e.code == qcb.EDEVICE
     else:
         c.close()
+
+
+def test_null_handles_rejected_without_a_device_call():
+    """A null handle is QCB_EINVAL before any HIP call, with or without a GPU."""
+    lib = qcb.lib()
+    out = ctypes.c_double()
+    assert lib.qcb_plaquette(None, ctypes.byref(out)) == qcb.EINVAL
+    assert lib.qcb_hb_sweeps(None, 6.0, 2, 1, 0, 0) == qcb.EINVAL
+    assert lib.qcb_flow_rk3(None, 0.02, 1) == qcb.EINVAL
+    assert lib.qcb_lattice_create(None, (ctypes.c_int * 4)(4, 4, 4, 4), qcb.F64, ctypes.byref(ctypes.c_void_p())) \
+        == qcb.EINVAL
+    assert lib.qcb_ctx_destroy(None) == qcb.EINVAL

@@ -75,8 +75,16 @@ def test_graph_replay_continues_counter(ctx):
     for x in (b, ref):
         x.hot()
     b.graph_capture(2, 2, first_sweep=5)
-    b.graph_launch(3)
-    ref.hb_sweeps(2, 6, sweep_idx=5)
+    # one replay at a time, with eager work and synchronisation between
+    # replays, and every replay checked against the eager sweeps
+    for r in range(3):
+        b.graph_launch(1)
+        ref.hb_sweeps(2, 2, sweep_idx=5 + 2 * r)
+        assert np.array_equal(b.plaquette(), ref.plaquette())
+        for i in range(len(DIMS)):
+            assert np.array_equal(b.lattice(i).get_links(), ref.lattice(i).get_links())
+    b.graph_launch(2)
+    ref.hb_sweeps(2, 4, sweep_idx=11)
     for i in range(len(DIMS)):
         assert np.array_equal(b.lattice(i).get_links(), ref.lattice(i).get_links())
 
