@@ -167,22 +167,28 @@ __device__ double compute_ds(const M3<double>& P, const M3<double>& R) {
     for (int k = 0; k < 3; ++k) {
 #pragma unroll
         for (int col = 0; col < 2; ++col) {
-            double mr = 0.0, mi = 0.0;
+            double mr = P.re(k, 0) * R.re(0, col), mi = P.re(k, 0) * R.im(0, col);
+            mr = fma(-P.im(k, 0), R.im(0, col), mr);
+            mi = fma(P.im(k, 0), R.re(0, col), mi);
 #pragma unroll
-            for (int l = 0; l < 3; ++l) {
-                mr += P.re(k, l) * R.re(l, col) - P.im(k, l) * R.im(l, col);
-                mi += P.re(k, l) * R.im(l, col) + P.im(k, l) * R.re(l, col);
+            for (int l = 1; l < 3; ++l) {
+                mr = fma(P.re(k, l), R.re(l, col), mr);
+                mr = fma(-P.im(k, l), R.im(l, col), mr);
+                mi = fma(P.re(k, l), R.im(l, col), mi);
+                mi = fma(P.im(k, l), R.re(l, col), mi);
             }
-            const double t = R.re(k, col) * mi - R.im(k, col) * mr;
-            res += col == 0 ? t : -t;
+            // Im(conj(R_k,col) (P R)_k,col), added for column 0 and subtracted for column 1
+            const double sg = col == 0 ? 1.0 : -1.0;
+            res = fma(sg * R.re(k, col), mi, res);
+            res = fma(-sg * R.im(k, col), mr, res);
         }
     }
     return res;
 }
 
-// update (ecmc_mpi_cb.cpp:357-372): U <- R diag(e^{i xi}, e^{-i xi}, 1) R^dag U
-__device__ M3<double> ecmc_update(const M3<double>& U, double theta, int eps, const M3<double>& R) {
-    M3<double> T = mul_dn(R, U);
+// update (ecmc_mpi_cb.cpp:357-372): U <- R diag(e^{i xi}, e^{-i xi}, 1) R^dag U,
+// given T = R^dag U (already formed for the rejection angles)
+__device__ M3<double> ecmc_update(M3<double> T, double theta, int eps, const M3<double>& R) {
     const double xi = eps * theta;
     double s, c;
     sincos(xi, &s, &c);
@@ -204,24 +210,41 @@ __device__ inline U4 ctr_ev(uint32_t chain, unsigned long long n, uint32_t call,
                          a.key1);
 }
 
-// random_SU3_epsilon (src/su3/utils.cpp:66-103): X01 X02 X12 near the identity
+// random_SU3_epsilon (src/su3/utils.cpp:66-103): X01 X02 X12 near the identity.
+// X_f is the identity outside rows/columns (i, j), so M <- M X_f changes
+// columns i and j only; the sums are those of the full product with its
+// exact zero terms left out, in the same order.
 __device__ M3<double> random_su3_eps(double eps, const double r[9]) {
     M3<double> M;
     m3_identity(M);
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        const int i = f == 2 ? 1 : 0, j = f == 0 ? 1 : 2, k = 3 - i - j;
+        const int i = f == 2 ? 1 : 0, j = f == 0 ? 1 : 2;
         const double n = sqrt(r[3 * f] * r[3 * f] + r[3 * f + 1] * r[3 * f + 1] + r[3 * f + 2] * r[3 * f + 2]);
         const double a0 = sqrt(1 - eps * eps), a1 = eps * r[3 * f] / n, a2 = eps * r[3 * f + 1] / n,
                      a3 = eps * r[3 * f + 2] / n;
-        M3<double> X;
-        m3_zero(X);
-        X.re(k, k) = 1.0;
-        X.re(i, i) = a0; X.im(i, i) = a3;
-        X.re(j, j) = a0; X.im(j, j) = -a3;
-        X.re(i, j) = a2; X.im(i, j) = a1;
-        X.re(j, i) = -a2; X.im(j, i) = a1;
-        M = mul_nn(M, X);
+        // X_ii = a0 + i a3, X_jj = a0 - i a3, X_ij = a2 + i a1, X_ji = -a2 + i a1
+#pragma unroll
+        for (int row = 0; row < 3; ++row) {
+            const double mir = M.re(row, i), mii = M.im(row, i), mjr = M.re(row, j), mji = M.im(row, j);
+            double cr = mir * a0, ci = mir * a3, dr = mir * a2, di = mir * a1;
+            cr = fma(-mii, a3, cr);
+            ci = fma(mii, a0, ci);
+            dr = fma(-mii, a1, dr);
+            di = fma(mii, a2, di);
+            cr = fma(mjr, -a2, cr);
+            ci = fma(mjr, a1, ci);
+            dr = fma(mjr, a0, dr);
+            di = fma(mjr, -a3, di);
+            cr = fma(-mji, a1, cr);
+            ci = fma(mji, -a2, ci);
+            dr = fma(-mji, -a3, dr);
+            di = fma(mji, a0, di);
+            M.re(row, i) = cr;
+            M.im(row, i) = ci;
+            M.re(row, j) = dr;
+            M.im(row, j) = di;
+        }
     }
     return M;
 }
@@ -253,15 +276,32 @@ __device__ inline int dir_of(int mu, int j) {
     return k + (k >= mu ? 1 : 0);
 }
 
+// a or a^dag, chosen per lane without a branch
+__device__ inline M3<double> cond_dag(const M3<double>& a, bool d) {
+    M3<double> o;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            o.re(i, j) = d ? a.re(j, i) : a.re(i, j);
+            o.im(i, j) = d ? -a.im(j, i) : a.im(i, j);
+        }
+    return o;
+}
+
+// Upper staple (j even) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag = [C B A^dag]^dag
+// with A = U_nu(x+mu), B = U_mu(x+nu), C = U_nu(x); lower staple (j odd)
+// U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = [C^dag B A]^dag with
+// A = U_nu(x+mu-nu), B = U_mu(x-nu), C = U_nu(x-nu).  Both are the same two
+// products of per-lane operands, so the six lanes of a team follow one path.
 __device__ M3<double> staple_j(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu, int j) {
     const int nu = dir_of(mu, j);
-    if ((j & 1) == 0) {
-        M3<double> t = mul_nn(ldl(L, a, x, nu), ldl(L, a, gsh(a, x, nu, 1), mu));
-        return mul_nd(ldl(L, a, gsh(a, x, mu, 1), nu), t);
-    }
-    const Site xmnu = gsh(a, x, nu, -1);
-    M3<double> t = mul_nn(ldl(L, a, xmnu, mu), ldl(L, a, gsh(a, x, mu, 1, nu, -1), nu));
-    return mul_dn(t, ldl(L, a, xmnu, nu));
+    const bool up = (j & 1) == 0;
+    const Site xb = gsh(a, x, nu, up ? 1 : -1);
+    const Site xa = gsh(a, x, mu, 1, nu, up ? 0 : -1);
+    const Site xc = up ? x : xb;
+    const M3<double> A = ldl(L, a, xa, nu), B = ldl(L, a, xb, mu), C = ldl(L, a, xc, nu);
+    return dag(mul_nn(mul_nn(cond_dag(C, !up), B), cond_dag(A, up)));
 }
 
 // the four links of plaquette j, the current link first (GeometryCB.cpp:60-90)
@@ -280,21 +320,46 @@ __device__ inline void plaq_links(const EcmcArgs& a, const Site& x, int mu, int 
     }
 }
 
-// plaquette matrix P_i of lift_improved_fast (ecmc_mpi_cb.cpp:296-318)
-__device__ M3<double> plaq_P(int i, bool fwd, const M3<double>& U0, const M3<double>& U1, const M3<double>& U2,
-                             const M3<double>& U3) {
-    if (fwd) {
-        const M3<double> U01 = mul_nn(U0, U1), U32 = mul_nn(U3, U2);
-        if (i == 0) return mul_nd(U01, U32);
-        if (i == 1) return mul_nn(mul_nd(U1, U32), U0);
-        if (i == 2) return mul_nn(mul_nd(U2, U01), U3);
-        return dag(mul_nd(U01, U32));
+// The plaquettes P_i of lift_improved_fast (ecmc_mpi_cb.cpp:296-318) are the
+// cyclic readings of one loop M = P_0 or of its inverse, each starting at
+// link i: P_i = V_i^dag M^(s_i) V_i with V_i unitary and M^(-) = M^dag.
+//   forward  (j even): M = U0 U1 U2^dag U3^dag,  V = 1, U0, U3, 1,
+//                      s = +, +, -, -
+//   backward (j odd):  M = U0 U1^dag U2^dag U3,  V = 1, U3^dag U2, U3^dag, U3^dag,
+//                      s = +, -, -, +
+// (e.g. forward P_2 = U3^dag M^dag U3 = U2 U1^dag U0^dag U3).  Since
+// (VR)^dag M^dag (VR) is the adjoint of (VR)^dag M (VR), compute_ds(P_i, R) =
+// s_i compute_ds(M, V_i R): every lane forms the same M and only its V_i
+// differs, so the four lanes follow one path.
+__device__ M3<double> plaq_loop(bool fwd, const M3<double>& U0, const M3<double>& U1, const M3<double>& U2,
+                                const M3<double>& U3) {
+    return mul_nn(mul_nd(mul_nn(U0, cond_dag(U1, !fwd)), U2), cond_dag(U3, fwd));
+}
+
+// columns 0 and 1 of V R (all that compute_ds reads of its second argument)
+__device__ M3<double> mul_cols01(const M3<double>& V, const M3<double>& R) {
+    M3<double> o;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            double re = V.re(i, 0) * R.re(0, j), im = V.re(i, 0) * R.im(0, j);
+            re = fma(-V.im(i, 0), R.im(0, j), re);
+            im = fma(V.im(i, 0), R.re(0, j), im);
+#pragma unroll
+            for (int k = 1; k < 3; ++k) {
+                re = fma(V.re(i, k), R.re(k, j), re);
+                re = fma(-V.im(i, k), R.im(k, j), re);
+                im = fma(V.re(i, k), R.im(k, j), im);
+                im = fma(V.im(i, k), R.re(k, j), im);
+            }
+            o.re(i, j) = re;
+            o.im(i, j) = im;
+        }
+        o.re(i, 2) = 0.0;
+        o.im(i, 2) = 0.0;
     }
-    const M3<double> T = mul_nd(U0, mul_nn(U2, U1));
-    if (i == 0) return mul_nn(T, U3);
-    if (i == 1) return mul_nn(mul_nd(mul_nd(U1, U0), U3), U2);
-    if (i == 3) return mul_nn(U3, T);
-    return dag(mul_nn(U3, T));
+    return o;
 }
 
 __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* __restrict__ lat,
@@ -372,21 +437,35 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         if (lane < 6) {
             const int j = lane;
             const M3<double> S = staple_j(L, a, xs, st.mu, j);
+            // (T S R)_00 and (T S R)_11 from rows 0, 1 of T S and columns 0, 1 of R
             double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
 #pragma unroll
             for (int col = 0; col < 3; ++col) {
-                double m0r = 0, m0i = 0, m1r = 0, m1i = 0;
+                double m0r = T.re(0, 0) * S.re(0, col), m0i = T.re(0, 0) * S.im(0, col);
+                double m1r = T.re(1, 0) * S.re(0, col), m1i = T.re(1, 0) * S.im(0, col);
+                m0r = fma(-T.im(0, 0), S.im(0, col), m0r);
+                m0i = fma(T.im(0, 0), S.re(0, col), m0i);
+                m1r = fma(-T.im(1, 0), S.im(0, col), m1r);
+                m1i = fma(T.im(1, 0), S.re(0, col), m1i);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    m0r += T.re(0, k) * S.re(k, col) - T.im(0, k) * S.im(k, col);
-                    m0i += T.re(0, k) * S.im(k, col) + T.im(0, k) * S.re(k, col);
-                    m1r += T.re(1, k) * S.re(k, col) - T.im(1, k) * S.im(k, col);
-                    m1i += T.re(1, k) * S.im(k, col) + T.im(1, k) * S.re(k, col);
+                for (int k = 1; k < 3; ++k) {
+                    m0r = fma(T.re(0, k), S.re(k, col), m0r);
+                    m0r = fma(-T.im(0, k), S.im(k, col), m0r);
+                    m0i = fma(T.re(0, k), S.im(k, col), m0i);
+                    m0i = fma(T.im(0, k), S.re(k, col), m0i);
+                    m1r = fma(T.re(1, k), S.re(k, col), m1r);
+                    m1r = fma(-T.im(1, k), S.im(k, col), m1r);
+                    m1i = fma(T.re(1, k), S.im(k, col), m1i);
+                    m1i = fma(T.im(1, k), S.re(k, col), m1i);
                 }
-                p00r += m0r * R.re(col, 0) - m0i * R.im(col, 0);
-                p00i += m0r * R.im(col, 0) + m0i * R.re(col, 0);
-                p11r += m1r * R.re(col, 1) - m1i * R.im(col, 1);
-                p11i += m1r * R.im(col, 1) + m1i * R.re(col, 1);
+                p00r = fma(m0r, R.re(col, 0), p00r);
+                p00r = fma(-m0i, R.im(col, 0), p00r);
+                p00i = fma(m0r, R.im(col, 0), p00i);
+                p00i = fma(m0i, R.re(col, 0), p00i);
+                p11r = fma(m1r, R.re(col, 1), p11r);
+                p11r = fma(-m1i, R.im(col, 1), p11r);
+                p11i = fma(m1r, R.im(col, 1), p11i);
+                p11i = fma(m1i, R.re(col, 1), p11i);
             }
             // gamma_j = -log u_j: lane j takes only its own exponential draw
             const uint32_t wj = j == 0 ? o0.x : j == 1 ? o0.y : j == 2 ? o0.z : j == 3 ? o0.w : j == 4 ? o1.x : o1.y;
@@ -415,23 +494,22 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         step = d_sample < step ? d_sample : step;
         step = d_rs < step ? d_rs : step;
         step = d_rR < step ? d_rR : step;
+        // every event kind moves the current link by the step angle first
+        // (ecmc_mpi_cb.cpp:571-628); the kinds differ only in their bookkeeping
+        const M3<double> U0 = ecmc_update(T, step, st.eps, R);
+        stl(L, a, xs, st.mu, U0);
+        st.events++;
         if (step == d_sample) {   // SAMPLE: the chain stops here until the next call
-            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_sample, st.eps, R));
-            st.events++;
             st.par_rs += d_sample;
             st.par_rR += d_sample;
             break;
         } else if (step == d_rs) {   // REFRESH SITE
-            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_rs, st.eps, R));
-            st.events++;
             par_sample += d_rs;
             st.par_rR += d_rs;
             st.par_rs = 0.0;
             const U4 o4 = draw_site(n);
             if (a.poisson) st.th_rs = -a.theta_refresh_site * log(u53(o4.z, o4.w));
         } else if (step == d_rR) {   // REFRESH R
-            stl(L, a, xs, st.mu, ecmc_update(Ucur, d_rR, st.eps, R));
-            st.events++;
             par_sample += d_rR;
             st.par_rs += d_rR;
             st.par_rR = 0.0;
@@ -439,9 +517,6 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             if (a.poisson) st.th_rR = -a.theta_refresh_R * log(u53(o5.x, o5.y));
             R = haar_su3_philox(a.key0, a.key1, chain, (uint32_t)n, 0xED000000u | ((uint32_t)(n >> 32) & 0x00FFFFFFu));
         } else {   // LIFT (lift_improved_fast, ecmc_mpi_cb.cpp:277-354)
-            const M3<double> U0 = ecmc_update(Ucur, theta_reject, st.eps, R);
-            stl(L, a, xs, st.mu, U0);
-            st.events++;
             par_sample += theta_reject;
             st.par_rs += theta_reject;
             st.par_rR += theta_reject;
@@ -461,9 +536,19 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
                 const int i = lane;
                 const Site si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
                 const int di = i == 0 ? ld[0] : i == 1 ? ld[1] : i == 2 ? ld[2] : ld[3];
-                const M3<double> P = plaq_P(i, fwd, U0, U1, U2, U3);
-                my_ds = frozen_near(a, org, si, di) ? 0.0 : compute_ds(P, R);
-                my_dsn = compute_ds(P, Rn);
+                const M3<double> M = plaq_loop(fwd, U0, U1, U2, U3);
+                const M3<double> U3d = dag(U3), Vb = mul_dn(U3, U2);
+                // V_i: 0 the identity, 1 U0, 2 U3, 3 U3^dag, 4 U3^dag U2
+                const int vk = fwd ? (i == 1 ? 1 : (i == 2 ? 2 : 0)) : (i == 0 ? 0 : (i == 1 ? 4 : 3));
+                const double sg = fwd ? (i < 2 ? 1.0 : -1.0) : ((i == 1 || i == 2) ? -1.0 : 1.0);
+                M3<double> V;
+#pragma unroll
+                for (int k = 0; k < 18; ++k) {
+                    const double id = (k == 0 || k == 8 || k == 16) ? 1.0 : 0.0;   // re of the diagonal
+                    V.v[k] = vk == 1 ? U0.v[k] : (vk == 2 ? U3.v[k] : (vk == 3 ? U3d.v[k] : (vk == 4 ? Vb.v[k] : id)));
+                }
+                my_ds = frozen_near(a, org, si, di) ? 0.0 : sg * compute_ds(M, mul_cols01(V, R));
+                my_dsn = sg * compute_ds(M, mul_cols01(V, Rn));
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
             double absd[4], sum = 0.0;
