@@ -43,6 +43,7 @@ struct EcmcArgs {
     uint32_t key0, key1;
     unsigned long long max_events;
     int n_chains;
+    uint32_t bvol;   // sites per block
 };
 
 constexpr uint64_t kSetSize = 10001;   // ecmc_set (src/su3/utils.cpp:119-127), 1 + 5000 pairs
@@ -67,61 +68,106 @@ __device__ inline uint32_t glx(const EcmcArgs& a, const int c[4]) {
     return (uint32_t)c[0] + (uint32_t)a.L[0] * ((uint32_t)c[1] + (uint32_t)a.L[1] * ((uint32_t)c[2] + (uint32_t)a.L[2] * (uint32_t)c[3]));
 }
 
-// Inside the event loop a site is carried as its coordinates: a neighbour is
-// a wrap of one coordinate and a link address a few small multiplies, so no
-// integer division is left in the loop (the lex index is formed only to
-// address memory and to keep the chain state).
+// Inside the event loop a site is carried as its offsets from the chain's
+// block origin.  Every site an event touches lies within [0, b_d] in each
+// direction: a non-frozen link (x, mu) has offsets 1..b_d-1 for d != mu and
+// 0..b_d-1 for d = mu, and its staples and plaquettes reach x+mu, x+-nu and
+// x+mu-nu only.  So a neighbour is one add with no wrap, and offset b_d is the
+// next block's face (offset 0 there).
 struct Site {
-    int c[4];
+    int o[4];
 };
 
-__device__ inline Site site_of(const EcmcArgs& a, uint32_t s) {
-    Site x;
-    gco(a, s, x.c);
-    return x;
-}
-
-__device__ inline Site gsh(const EcmcArgs& a, Site x, int d1, int s1, int d2 = 0, int s2 = 0) {
+__device__ inline Site gsh(Site x, int d1, int s1, int d2 = 0, int s2 = 0) {
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        const int sh = (d == d1 ? s1 : 0) + (d == d2 ? s2 : 0);
-        if (sh) {
-            int v = x.c[d] + sh;
-            v = v < 0 ? v + a.L[d] : (v >= a.L[d] ? v - a.L[d] : v);
-            x.c[d] = v;
-        }
-    }
+    for (int d = 0; d < 4; ++d) x.o[d] += (d == d1 ? s1 : 0) + (d == d2 ? s2 : 0);
     return x;
 }
 
 __device__ inline bool same_site(const Site& x, const Site& y) {
-    return x.c[0] == y.c[0] && x.c[1] == y.c[1] && x.c[2] == y.c[2] && x.c[3] == y.c[3];
+    return x.o[0] == y.o[0] && x.o[1] == y.o[1] && x.o[2] == y.o[2] && x.o[3] == y.o[3];
 }
 
-// frozen (x_d mod b_d == 0 for some d != mu) for the sites an event touches:
-// they lie within [org, org + b] of the chain's block origin per direction
-// (modulo L), where x_d mod b_d == 0 means an offset of 0 or b
-__device__ inline bool frozen_near(const EcmcArgs& a, const int org[4], const Site& x, int mu) {
+// frozen: x_d mod b_d == 0 for some d != mu, i.e. an offset of 0 or b_d
+__device__ inline bool frozen_near(const EcmcArgs& a, const Site& x, int mu) {
     bool f = false;
 #pragma unroll
-    for (int d = 0; d < 4; ++d) {
-        int r = x.c[d] - org[d];
-        r = r < 0 ? r + a.L[d] : r;
-        f = f || (d != mu && (r == 0 || r == a.b[d]));
-    }
+    for (int d = 0; d < 4; ++d) f = f || (d != mu && (x.o[d] == 0 || x.o[d] == a.b[d]));
     return f;
 }
 
-// links by site: cb index = lex / 2 (Lx even), parity = coordinate sum
-__device__ inline M3<double> ldl(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu) {
-    return ld_link(L.U, L.S, mu, (x.c[0] + x.c[1] + x.c[2] + x.c[3]) & 1, glx(a, x.c) >> 1);
+// lexicographic index of a site given by its offsets from origin org
+__device__ inline uint32_t lex_of(const EcmcArgs& a, const int org[4], const Site& x) {
+    int c[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        c[d] = org[d] + x.o[d];
+        c[d] = c[d] >= a.L[d] ? c[d] - a.L[d] : c[d];
+    }
+    return glx(a, c);
 }
 
-__device__ inline void stl(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu, const M3<double>& m) {
-    double2* b = (double2*)L.U + link_base<double>(L.S, mu * 2 + ((x.c[0] + x.c[1] + x.c[2] + x.c[3]) & 1),
-                                                   glx(a, x.c) >> 1);
+// The chains work on a chain-local copy of the links: block after block
+// (block index = chain index), a block's sites in lexicographic order, the
+// four links of a site, 18 doubles per link.  A link is 144 contiguous bytes
+// and a chain's links stay within its block's b^4 * 576 bytes, so the links
+// an event touches share cache lines and pages.
+__device__ inline size_t blk_link(const EcmcArgs& a, const int bc[4], const Site& x, int mu) {
+    uint32_t blk = 0, loc = 0;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) b[64 * k] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
+    for (int d = 3; d >= 0; --d) {
+        const bool nx = x.o[d] == a.b[d];
+        int bd = bc[d] + (nx ? 1 : 0);
+        bd = bd == a.nb[d] ? 0 : bd;
+        blk = blk * (uint32_t)a.nb[d] + (uint32_t)bd;
+        loc = loc * (uint32_t)a.b[d] + (uint32_t)(nx ? 0 : x.o[d]);
+    }
+    return (((size_t)blk * a.bvol + loc) * 4 + (size_t)mu) * 18;
+}
+
+__device__ inline M3<double> ldl(const double* __restrict__ E, const EcmcArgs& a, const int bc[4], const Site& x,
+                                 int mu) {
+    const double2* p = (const double2*)(E + blk_link(a, bc, x, mu));
+    M3<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double2 v = p[k];
+        m.v[2 * k] = v.x;
+        m.v[2 * k + 1] = v.y;
+    }
+    return m;
+}
+
+__device__ inline void stl(double* __restrict__ E, const EcmcArgs& a, const int bc[4], const Site& x, int mu,
+                           const M3<double>& m) {
+    double2* p = (double2*)(E + blk_link(a, bc, x, mu));
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p[k] = make_double2(m.v[2 * k], m.v[2 * k + 1]);
+}
+
+// Copy the lattice's links (even/odd AoSoA) into the chain-local copy, or
+// back; one thread per link, in the copy's order.
+__global__ void k_ecmc_pack(const LatDev<double>* __restrict__ lat, double* __restrict__ E, EcmcArgs a, int back) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint32_t)a.n_chains * a.bvol * 4u) return;
+    const LatDev<double>& L = *lat;
+    const int mu = (int)(t & 3u);
+    uint32_t loc = (t >> 2) % a.bvol, blk = (t >> 2) / a.bvol;
+    int c[4];
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        c[d] = (int)(blk % (uint32_t)a.nb[d]) * a.b[d] + (int)(loc % (uint32_t)a.b[d]);
+        blk /= (uint32_t)a.nb[d];
+        loc /= (uint32_t)a.b[d];
+    }
+    const int par = (c[0] + c[1] + c[2] + c[3]) & 1;
+    double2* u = (double2*)L.U + link_base<double>(L.S, mu * 2 + par, glx(a, c) >> 1);
+    double2* e = (double2*)(E + (size_t)t * 18);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        if (back) u[64 * k] = e[k];
+        else e[k] = u[64 * k];
+    }
 }
 
 __device__ inline M3<double> dag(const M3<double>& a) {
@@ -294,28 +340,29 @@ __device__ inline M3<double> cond_dag(const M3<double>& a, bool d) {
 // U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = [C^dag B A]^dag with
 // A = U_nu(x+mu-nu), B = U_mu(x-nu), C = U_nu(x-nu).  Both are the same two
 // products of per-lane operands, so the six lanes of a team follow one path.
-__device__ M3<double> staple_j(const LatDev<double>& L, const EcmcArgs& a, const Site& x, int mu, int j) {
+__device__ M3<double> staple_j(const double* __restrict__ E, const EcmcArgs& a, const int bc[4], const Site& x, int mu,
+                               int j) {
     const int nu = dir_of(mu, j);
     const bool up = (j & 1) == 0;
-    const Site xb = gsh(a, x, nu, up ? 1 : -1);
-    const Site xa = gsh(a, x, mu, 1, nu, up ? 0 : -1);
+    const Site xb = gsh(x, nu, up ? 1 : -1);
+    const Site xa = gsh(x, mu, 1, nu, up ? 0 : -1);
     const Site xc = up ? x : xb;
-    const M3<double> A = ldl(L, a, xa, nu), B = ldl(L, a, xb, mu), C = ldl(L, a, xc, nu);
+    const M3<double> A = ldl(E, a, bc, xa, nu), B = ldl(E, a, bc, xb, mu), C = ldl(E, a, bc, xc, nu);
     return dag(mul_nn(mul_nn(cond_dag(C, !up), B), cond_dag(A, up)));
 }
 
 // the four links of plaquette j, the current link first (GeometryCB.cpp:60-90)
-__device__ inline void plaq_links(const EcmcArgs& a, const Site& x, int mu, int j, Site s[4], int d[4]) {
+__device__ inline void plaq_links(const Site& x, int mu, int j, Site s[4], int d[4]) {
     const int nu = dir_of(mu, j);
     s[0] = x;
     d[0] = mu;
     if ((j & 1) == 0) {
-        s[1] = gsh(a, x, mu, 1); d[1] = nu;
-        s[2] = gsh(a, x, nu, 1); d[2] = mu;
+        s[1] = gsh(x, mu, 1); d[1] = nu;
+        s[2] = gsh(x, nu, 1); d[2] = mu;
         s[3] = x;                d[3] = nu;
     } else {
-        s[1] = gsh(a, x, mu, 1, nu, -1); d[1] = nu;
-        s[2] = gsh(a, x, nu, -1);        d[2] = mu;
+        s[1] = gsh(x, mu, 1, nu, -1); d[1] = nu;
+        s[2] = gsh(x, nu, -1);        d[2] = mu;
         s[3] = s[2];                     d[3] = nu;
     }
 }
@@ -362,36 +409,41 @@ __device__ M3<double> mul_cols01(const M3<double>& V, const M3<double>& R) {
     return o;
 }
 
-__global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* __restrict__ lat,
-                                                          EcmcChainDev* __restrict__ chains, EcmcArgs a) {
+__global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E, EcmcChainDev* __restrict__ chains,
+                                                          EcmcArgs a) {
     const int tid = blockIdx.x * kThreads + threadIdx.x;
     const int cid = tid / kTeam, lane = tid % kTeam;
     if (cid >= a.n_chains) return;   // whole teams leave together
-    const LatDev<double>& L = *lat;
     const uint32_t chain = (uint32_t)cid;
     EcmcChainDev st = chains[cid];
-    int org[4];
+    int org[4], bc[4];   // block origin and block coordinates (block index = chain index)
     {
         int r = cid;
 #pragma unroll
         for (int d = 0; d < 4; ++d) {
-            org[d] = (r % a.nb[d]) * a.b[d];
+            bc[d] = r % a.nb[d];
+            org[d] = bc[d] * a.b[d];
             r /= a.nb[d];
         }
     }
     M3<double> R;
 #pragma unroll
     for (int k = 0; k < 18; ++k) R.v[k] = st.R[k];
-    Site xs = site_of(a, st.site);   // the chain's current site
+    Site xs;   // the chain's current site
+    {
+        int c[4];
+        gco(a, st.site, c);
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int r = c[d] - org[d];
+            xs.o[d] = r < 0 ? r + a.L[d] : r;
+        }
+    }
     auto draw_site = [&](unsigned long long n) {
         const U4 o3 = ctr_ev(chain, n, 3, a), o4 = ctr_ev(chain, n, 4, a);
         const uint32_t w[4] = {o3.x, o3.y, o3.z, o3.w};
-        int c[4];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) c[d] = org[d] + 1 + (int)__umulhi(w[d], (uint32_t)(a.b[d] - 2));
-        st.site = glx(a, c);
-#pragma unroll
-        for (int d = 0; d < 4; ++d) xs.c[d] = c[d];
+        for (int d = 0; d < 4; ++d) xs.o[d] = 1 + (int)__umulhi(w[d], (uint32_t)(a.b[d] - 2));
         st.mu = (int)(o4.x >> 30);
         st.eps = 2 * (int)(o4.y >> 31) - 1;
         return o4;
@@ -428,7 +480,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         n += 1;
         const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
-        const M3<double> Ucur = ldl(L, a, xs, st.mu);
+        const M3<double> Ucur = ldl(E, a, bc, xs, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
         // lane j < 6 solves staple j; the team takes the smallest angle, the
         // lowest j on ties (the sequential scan's first minimum)
@@ -436,7 +488,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         int jmin = lane;
         if (lane < 6) {
             const int j = lane;
-            const M3<double> S = staple_j(L, a, xs, st.mu, j);
+            const M3<double> S = staple_j(E, a, bc, xs, st.mu, j);
             // (T S R)_00 and (T S R)_11 from rows 0, 1 of T S and columns 0, 1 of R
             double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
 #pragma unroll
@@ -497,7 +549,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
         // every event kind moves the current link by the step angle first
         // (ecmc_mpi_cb.cpp:571-628); the kinds differ only in their bookkeeping
         const M3<double> U0 = ecmc_update(T, step, st.eps, R);
-        stl(L, a, xs, st.mu, U0);
+        stl(E, a, bc, xs, st.mu, U0);
         st.events++;
         if (step == d_sample) {   // SAMPLE: the chain stops here until the next call
             st.par_rs += d_sample;
@@ -522,8 +574,9 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             st.par_rR += theta_reject;
             Site ls[4];
             int ld[4];
-            plaq_links(a, xs, st.mu, jmin, ls, ld);
-            const M3<double> U1 = ldl(L, a, ls[1], ld[1]), U2 = ldl(L, a, ls[2], ld[2]), U3 = ldl(L, a, ls[3], ld[3]);
+            plaq_links(xs, st.mu, jmin, ls, ld);
+            const M3<double> U1 = ldl(E, a, bc, ls[1], ld[1]), U2 = ldl(E, a, bc, ls[2], ld[2]),
+                             U3 = ldl(E, a, bc, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
             // the proposed R' does not depend on the chosen plaquette, so it is
             // formed first and lane i < 4 evaluates plaquette i against both R
@@ -547,7 +600,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
                     const double id = (k == 0 || k == 8 || k == 16) ? 1.0 : 0.0;   // re of the diagonal
                     V.v[k] = vk == 1 ? U0.v[k] : (vk == 2 ? U3.v[k] : (vk == 3 ? U3d.v[k] : (vk == 4 ? Vb.v[k] : id)));
                 }
-                my_ds = frozen_near(a, org, si, di) ? 0.0 : sg * compute_ds(M, mul_cols01(V, R));
+                my_ds = frozen_near(a, si, di) ? 0.0 : sg * compute_ds(M, mul_cols01(V, R));
                 my_dsn = sg * compute_ds(M, mul_cols01(V, Rn));
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
@@ -577,11 +630,11 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(const LatDev<double>* 
             st.set_counter++;
             st.lifts++;
             xs = xn;
-            st.site = glx(a, xs.c);
             st.mu = ld[idx];
             st.eps = (int)neps;
         }
     }
+    st.site = lex_of(a, org, xs);
     st.rng_n = n;
     st.set_gen = gen;
 #pragma unroll
@@ -598,6 +651,7 @@ struct qcb_ecmc {
     EcmcArgs args{};
     int n_chains = 0;
     EcmcChainDev* d_chains = nullptr;
+    double* d_links = nullptr;   // chain-local copy of the lattice's links (FP64, 72 B x 4 V)
 };
 
 extern "C" int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_ecmc** out) {
@@ -628,15 +682,20 @@ extern "C" int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_e
     a.key1 = (uint32_t)(p->seed >> 32);
     a.max_events = p->max_events ? p->max_events : 1000000000ull;
     a.n_chains = n;
+    a.bvol = (uint32_t)(a.b[0] * a.b[1] * a.b[2] * a.b[3]);
     qcb_ecmc* e = new qcb_ecmc();
     e->lat = lat;
     e->args = a;
     e->n_chains = n;
+    const size_t link_bytes = (size_t)lat->V * 4 * 18 * sizeof(double);
     if (hipMalloc((void**)&e->d_chains, sizeof(EcmcChainDev) * n) != hipSuccess ||
-        hipMemset(e->d_chains, 0, sizeof(EcmcChainDev) * n) != hipSuccess) {
+        hipMemset(e->d_chains, 0, sizeof(EcmcChainDev) * n) != hipSuccess ||
+        hipMalloc((void**)&e->d_links, link_bytes) != hipSuccess) {
         if (e->d_chains) (void)hipFree(e->d_chains);
+        if (e->d_links) (void)hipFree(e->d_links);
         delete e;
-        return qcb_set_error(ctx, QCB_ENOMEM, "cannot allocate %d chain states", n);
+        return qcb_set_error(ctx, QCB_ENOMEM, "cannot allocate %d chain states and a %zu-byte link copy", n,
+                             link_bytes);
     }
     ctx->n_live++;
     *out = e;
@@ -648,6 +707,7 @@ extern "C" int qcb_ecmc_destroy(qcb_ecmc* e) {
     qcb_ctx* ctx = e->lat->ctx;
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(e->d_chains);
+    (void)hipFree(e->d_links);
     ctx->n_live--;
     delete e;
     return QCB_OK;
@@ -663,10 +723,16 @@ extern "C" int qcb_ecmc_sample(qcb_ecmc* e, int n_samples) {
     if (!e || n_samples < 0) return QCB_EINVAL;
     qcb_lattice* lat = e->lat;
     qcb_ctx* ctx = lat->ctx;
+    if (n_samples == 0) return QCB_OK;
     const int blocks = (e->n_chains * kTeam + kThreads - 1) / kThreads;
+    const uint32_t n_links = (uint32_t)lat->V * 4u;
+    const dim3 pack_grid((n_links + 255u) / 256u);
+    const LatDev<double>* desc = (const LatDev<double>*)lat->d_desc;
+    hipLaunchKernelGGL(k_ecmc_pack, pack_grid, dim3(256), 0, ctx->stream, desc, e->d_links, e->args, 0);
     for (int s = 0; s < n_samples; ++s)
-        hipLaunchKernelGGL(k_ecmc_sample, dim3(blocks), dim3(kThreads), 0, ctx->stream,
-                           (const LatDev<double>*)lat->d_desc, e->d_chains, e->args);
+        hipLaunchKernelGGL(k_ecmc_sample, dim3(blocks), dim3(kThreads), 0, ctx->stream, e->d_links, e->d_chains,
+                           e->args);
+    hipLaunchKernelGGL(k_ecmc_pack, pack_grid, dim3(256), 0, ctx->stream, desc, e->d_links, e->args, 1);
     QCB_HIP(ctx, hipGetLastError());
     return QCB_OK;
 }
