@@ -479,6 +479,12 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
         }
         n += 1;
         const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
+        // the lift's proposed R' depends on neither the links nor the chosen
+        // plaquette: forming it here lets it overlap the event's link loads
+        // (it is used only if the event is a lift, where R is still this R)
+        const U4 o2 = ctr_ev(chain, n, 2, a);
+        const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
+        const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
         const M3<double> Ucur = ldl(E, a, bc, xs, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
@@ -578,12 +584,8 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             const M3<double> U1 = ldl(E, a, bc, ls[1], ld[1]), U2 = ldl(E, a, bc, ls[2], ld[2]),
                              U3 = ldl(E, a, bc, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
-            // the proposed R' does not depend on the chosen plaquette, so it is
-            // formed first and lane i < 4 evaluates plaquette i against both R
-            // and R'; every lane then holds all four dS and dS'
-            const U4 o2 = ctr_ev(chain, n, 2, a);
-            const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
-            const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
+            // lane i < 4 evaluates plaquette i against both R and R'; every
+            // lane then holds all four dS and dS'
             double my_ds = 0.0, my_dsn = 0.0;
             if (lane < 4) {
                 const int i = lane;
