@@ -239,7 +239,9 @@ typedef struct {
                                      par_refresh_R, events, lifts, revs, set_counter, set_gen,
                                      rng_n, site (lex), mu, epsilon, initialized */
 /* mpi::ecmccb::sample_persistant chain states for every block of lat (chain c
- * is block c in x-fastest block order); lat must outlive the handle */
+ * is block c in x-fastest block order); lat must outlive the handle.  The
+ * handle also holds a chain-local copy of lat's links (72 B per link, FP64),
+ * filled and written back by every qcb_ecmc_sample call */
 int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_ecmc** out);
 int qcb_ecmc_destroy(qcb_ecmc* e);
 int qcb_ecmc_n_chains(qcb_ecmc* e, int* n);
