@@ -7,7 +7,7 @@
 // and is cut into the same blocks: a link (x, mu) is frozen when
 // x_d mod b_d == 0 for some d != mu, a non-frozen link of a block only reads
 // links of its own block or frozen ones, so every block's chain runs at once,
-// one chain per team of 8 lanes.  Chains keep their LocalChainState between samples
+// one chain per team of 16 lanes.  Chains keep their LocalChainState between samples
 // (sample_persistant) in device memory.  Randomness is Philox4x32-10 on
 // per-chain counters (scheme in include/qcb.h), so each chain's trajectory
 // depends only on (seed, chain index, its own history).  FP64 links.
@@ -47,11 +47,17 @@ struct EcmcArgs {
 };
 
 constexpr uint64_t kSetSize = 10001;   // ecmc_set (src/su3/utils.cpp:119-127), 1 + 5000 pairs
-// A chain is worked by a team of 8 lanes: lanes 0-5 form and solve the six
+// A chain is worked by a team of 16 lanes: lanes 0-5 form and solve the six
 // staples of an event in parallel, lanes 0-3 the four plaquettes of a lift;
 // the rest of the event is computed identically by every lane of the team.
-constexpr int kTeam = 8;
-constexpr int kThreads = 64;   // one wavefront per block: 8 chains
+// A wavefront's teams run one instruction stream, so more chains per
+// wavefront share its instructions, while fewer leave less to run one after
+// the other where they branch apart and let more wavefronts share the SIMDs:
+// four chains per wavefront put the 4096 chains of a 32^4 lattice in 4^4
+// blocks on all 1024 SIMDs, one wavefront each (5% more events/s than eight
+// chains per wavefront on half of the SIMDs).
+constexpr int kTeam = 16;
+constexpr int kThreads = 64;   // one wavefront per block: 4 chains
 
 __device__ inline int dsgn(double x) { return x > 0 ? 1 : (x < 0 ? -1 : 0); }
 
