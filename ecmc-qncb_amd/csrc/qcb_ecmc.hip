@@ -346,14 +346,19 @@ __device__ inline M3<double> cond_dag(const M3<double>& a, bool d) {
 // U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu) = [C^dag B A]^dag with
 // A = U_nu(x+mu-nu), B = U_mu(x-nu), C = U_nu(x-nu).  Both are the same two
 // products of per-lane operands, so the six lanes of a team follow one path.
-__device__ M3<double> staple_j(const double* __restrict__ E, const EcmcArgs& a, const int bc[4], const Site& x, int mu,
-                               int j) {
+__device__ inline void staple_links(const double* __restrict__ E, const EcmcArgs& a, const int bc[4], const Site& x,
+                                    int mu, int j, M3<double>& A, M3<double>& B, M3<double>& C) {
     const int nu = dir_of(mu, j);
     const bool up = (j & 1) == 0;
     const Site xb = gsh(x, nu, up ? 1 : -1);
     const Site xa = gsh(x, mu, 1, nu, up ? 0 : -1);
     const Site xc = up ? x : xb;
-    const M3<double> A = ldl(E, a, bc, xa, nu), B = ldl(E, a, bc, xb, mu), C = ldl(E, a, bc, xc, nu);
+    A = ldl(E, a, bc, xa, nu);
+    B = ldl(E, a, bc, xb, mu);
+    C = ldl(E, a, bc, xc, nu);
+}
+
+__device__ inline M3<double> staple_of(bool up, const M3<double>& A, const M3<double>& B, const M3<double>& C) {
     return dag(mul_nn(mul_nn(cond_dag(C, !up), B), cond_dag(A, up)));
 }
 
@@ -484,6 +489,12 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             st.set_counter = 0;
         }
         n += 1;
+        // the event's link loads come first, for every lane (lanes past 5
+        // repeat staple 5's addresses and discard the result), so that their
+        // latency overlaps the random numbers and R' below
+        M3<double> SA, SB, SC;
+        staple_links(E, a, bc, xs, st.mu, lane < 6 ? lane : 5, SA, SB, SC);
+        const M3<double> Ucur = ldl(E, a, bc, xs, st.mu);
         const U4 o0 = ctr_ev(chain, n, 0, a), o1 = ctr_ev(chain, n, 1, a);
         // the lift's proposed R' depends on neither the links nor the chosen
         // plaquette: forming it here lets it overlap the event's link loads
@@ -492,7 +503,6 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
         const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
         const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
-        const M3<double> Ucur = ldl(E, a, bc, xs, st.mu);
         const M3<double> T = mul_dn(R, Ucur);
         // lane j < 6 solves staple j; the team takes the smallest angle, the
         // lowest j on ties (the sequential scan's first minimum)
@@ -500,7 +510,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
         int jmin = lane;
         if (lane < 6) {
             const int j = lane;
-            const M3<double> S = staple_j(E, a, bc, xs, st.mu, j);
+            const M3<double> S = staple_of((j & 1) == 0, SA, SB, SC);
             // (T S R)_00 and (T S R)_11 from rows 0, 1 of T S and columns 0, 1 of R
             double p00r = 0, p00i = 0, p11r = 0, p11i = 0;
 #pragma unroll
