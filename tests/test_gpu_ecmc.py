@@ -41,9 +41,10 @@ def _compare(chains_gpu, chains_or):
         assert abs(c[F["th_refresh_site"]] - s.th_refresh_site) < 1e-8
 
 
-@pytest.mark.parametrize("poisson,block", [(False, (4, 4, 4, 4)), (True, (4, 4, 8, 4))])
-def test_chains_match_oracle(ctx, poisson, block):
-    dims = (8, 8, 8, 8)
+# the last case has 6 chains (a wavefront holds 4) and one block along z and t
+@pytest.mark.parametrize("poisson,dims,block", [(False, (8, 8, 8, 8), (4, 4, 4, 4)), (True, (8, 8, 8, 8), (4, 4, 8, 4)),
+                                                (False, (12, 8, 4, 8), (4, 4, 4, 8))])
+def test_chains_match_oracle(ctx, poisson, dims, block):
     g, o = _pair(ctx, dims, 3)
     # budgets with no common multiple: with fixed budgets 12 and 3 the sample
     # event and a refresh-R event coincide exactly, and rounding picks one
