@@ -787,6 +787,10 @@ extern "C" int qcb_ecmc_set_chains(qcb_ecmc* e, const double* in) {
     std::vector<EcmcChainDev> h(e->n_chains);
     for (int c = 0; c < e->n_chains; ++c) {
         const double* v = in + (size_t)c * QCB_ECMC_CHAIN_DOUBLES;
+        // counters and the site are whole numbers carried as doubles
+        auto whole = [](double x) { return x >= 0 && x < 9007199254740992.0 && x == (double)(uint64_t)x; };
+        for (int k = 22; k <= 28; ++k)
+            if (!whole(v[k])) return qcb_set_error(ctx, QCB_EINVAL, "chain %d: field %d is not a count", c, k);
         EcmcChainDev& s = h[c];
         std::memset(&s, 0, sizeof s);
         for (int k = 0; k < 18; ++k) s.R[k] = v[k];
@@ -794,13 +798,16 @@ extern "C" int qcb_ecmc_set_chains(qcb_ecmc* e, const double* in) {
         s.events = (unsigned long long)v[22]; s.lifts = (unsigned long long)v[23]; s.revs = (unsigned long long)v[24];
         s.set_counter = (unsigned long long)v[25]; s.set_gen = (unsigned long long)v[26];
         s.rng_n = (unsigned long long)v[27];
-        s.mu = (int)v[29]; s.eps = (int)v[30]; s.initialized = v[31] != 0.0;
+        s.initialized = v[31] != 0.0;
         // the site must lie in the chain's own block (or be unset for a fresh chain)
         const double site = v[28];
         const uint64_t V = (uint64_t)a.L[0] * a.L[1] * a.L[2] * a.L[3];
         if (s.initialized) {
-            if (!(site >= 0 && site < (double)V) || s.mu < 0 || s.mu > 3 || (s.eps != 1 && s.eps != -1))
+            if (!(site < (double)V) || !(v[29] == 0 || v[29] == 1 || v[29] == 2 || v[29] == 3) ||
+                !(v[30] == 1.0 || v[30] == -1.0))
                 return qcb_set_error(ctx, QCB_EINVAL, "chain %d: site/mu/epsilon out of range", c);
+            s.mu = (int)v[29];
+            s.eps = (int)v[30];
             uint32_t x = (uint32_t)site;
             int r = c;
             for (int d = 0; d < 4; ++d) {

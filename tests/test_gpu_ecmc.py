@@ -138,10 +138,12 @@ def test_set_chains_round_trip_and_validation(ctx):
     bad[0, F["site"]] = sum(v * 8 ** d for d, v in enumerate(x))
     with pytest.raises(qcb.QcbError):
         e2.set_chains(bad)
-    bad = st.copy()
-    bad[0, F["eps"]] = 0
-    with pytest.raises(qcb.QcbError):
-        e2.set_chains(bad)
+    for field, value in (("eps", 0), ("mu", 1.5), ("mu", 4), ("site", st[0, F["site"]] + 0.5), ("events", -1),
+                         ("lifts", 2.0 ** 60), ("rng_n", float("nan"))):
+        bad = st.copy()
+        bad[0, F[field]] = value
+        with pytest.raises(qcb.QcbError):
+            e2.set_chains(bad)
     assert np.array_equal(e2.chains(), want_chains)   # refused states change nothing
     e.close()
     e2.close()
