@@ -112,6 +112,8 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
                                 const uint64_t* seeds, qcb_batch** out) {
     if (!ctx || n < 1 || !dims || !betas || !seeds || !out) return QCB_EINVAL;
     *out = nullptr;
+    for (int i = 0; i < n; ++i)
+        if (!(betas[i] > 0)) return qcb_set_error(ctx, QCB_EINVAL, "lattice %d: need beta > 0", i);
     qcb_batch* b = new qcb_batch();
     b->ctx = ctx;
     b->prec = prec;

@@ -234,9 +234,13 @@ extern "C" double qcb_service_clock(qcb_service* s) { return s ? now_s(s) : 0.0;
 
 extern "C" int qcb_service_submit(qcb_service* s, uint64_t id, const int dims[4], double beta, uint64_t seed,
                                   double t_max, double t_arrival) {
-    if (!s || !dims || !(beta > 0) || t_max < 0) return QCB_EINVAL;
+    if (!s || !dims) return QCB_EINVAL;
+    if (!(beta > 0) || !(t_max >= 0) || !(t_max / s->eps <= 1e8))
+        return qcb_set_error(s->ctx, QCB_EINVAL, "need beta > 0 and 0 <= t_max <= 1e8 eps");
     for (int d = 0; d < 4; ++d)
         if (dims[d] < 4 || dims[d] % 2) return qcb_set_error(s->ctx, QCB_EINVAL, "extents must be even and >= 4");
+    if ((uint64_t)dims[0] * dims[1] * dims[2] * dims[3] * 4 >= (1ull << 31))
+        return qcb_set_error(s->ctx, QCB_EINVAL, "lattice too large");
     Req* r = new Req();
     r->id = id;
     qcb_lattice& l = r->lat;

@@ -99,3 +99,20 @@ def test_context_destroy_with_live_handles_is_refused(ctx):
     c.close()                             # closes the lattice first, then the context
     assert not c.h
 
+
+
+def test_batch_and_service_arguments_rejected(ctx):
+    code, msg = code_of(lambda: qcb.Batch(ctx, [(4, 4, 4, 4), (6, 6, 6, 6)], qcb.F32, [6.0, 0.0], [1, 2]))
+    assert code == qcb.EINVAL and "lattice 1: need beta > 0" in msg
+    code, msg = code_of(lambda: qcb.Batch(ctx, [(4, 4, 4, 4), (6, 5, 6, 6)], qcb.F32, [6.0, 6.0], [1, 2]))
+    assert code == qcb.EINVAL and "even and >= 4" in msg
+    s = qcb.Service(ctx, arena_bytes=64 << 20)
+    for dims, beta, t_max in (((12, 12, 12, 24), 6.0, float("nan")), ((12, 12, 12, 24), 0.0, 1.0),
+                              ((12, 12, 12, 24), 6.0, -1.0), ((12, 12, 12, 24), 6.0, 1e9),
+                              ((12, 12, 13, 24), 6.0, 1.0), ((256, 256, 256, 256), 6.0, 1.0)):
+        code, _ = code_of(lambda: s.submit(1, dims, beta, 7, t_max))
+        assert code == qcb.EINVAL, (dims, beta, t_max)
+    # a request larger than the whole arena is refused as such
+    code, msg = code_of(lambda: s.submit(2, (24, 24, 24, 48), 6.0, 7, 1.0))
+    assert code == qcb.ENOMEM and "arena" in msg
+    s.close()
