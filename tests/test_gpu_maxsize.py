@@ -1,0 +1,67 @@
+"""A 128^4 FP32 lattice (268M sites, 4V = 2^30 links; with its flow buffers
+~200 GB of the device's HBM): the largest size class the ABI admits (4V must
+stay below 2^31) that also fits the flow buffers.  Checked through
+size-independent properties only, with no host copy of the links: the cold
+field's closed forms, the hot start's plaquette near zero, a heatbath sweep
+and a flow step against the same operations at 16^4, translation invariance
+of the plaquette and of Q and E, and reunitarisation leaving SU(3) links
+alone."""
+import pytest
+
+import qcb
+
+pytestmark = pytest.mark.gpu
+
+BIG = (128, 128, 128, 128)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = qcb.Context(0)
+    yield c
+    c.close()
+
+
+def test_largest_fp32_lattice(ctx):
+    print(f"max-size lattice {BIG} FP32", flush=True)
+    g = qcb.Lattice(ctx, BIG, qcb.F32)
+    assert g.plaquette() == pytest.approx(1.0, abs=1e-12)
+    q, e = g.clover_qe()
+    assert abs(q) < 1e-9 and abs(e) < 1e-12
+    g.hot(11)
+    p_hot = g.plaquette()
+    assert abs(p_hot) < 1e-3                     # Haar links: <P> = 0 with std ~ 1e-5 here
+    g.hb_sweeps(6.0, 2, 1, seed=3)
+    p1 = g.plaquette()
+    # the same sweep on a 16^4 lattice: one sweep from a random start moves <P>
+    # to a value set by beta, not by the volume
+    s = qcb.Lattice(ctx, (16, 16, 16, 16), qcb.F32)
+    s.hot(11)
+    s.hb_sweeps(6.0, 2, 1, seed=3)
+    assert abs(p1 - s.plaquette()) < 5e-3 and p1 > 0.1
+    # translation invariance of the observables (different summation order only)
+    q1, e1 = g.clover_qe()
+    g.translate((1, 0, 0, 0))
+    g.translate((0, 0, 0, -1))
+    assert abs(g.plaquette() - p1) < 1e-10
+    q2, e2 = g.clover_qe()
+    # per-site values are bitwise the same; only the FP64 sums change order
+    assert abs(q2 - q1) < 1e-6 and abs(e2 - e1) < 1e-9 * e1
+    # the stored links are SU(3) to FP32 rounding: reunitarisation barely moves them
+    g.reunitarize()
+    assert abs(g.plaquette() - p1) < 1e-6
+    # one flow step changes the clover energy by the same factor as at 16^4
+    # (one sweep from a random start is far from smooth: the step raises the
+    # clover E while it lowers the Wilson action)
+    g.flow_copy()
+    _, e_before = g.flow_qe()
+    g.flow_rk3(0.02, 1)
+    _, e_after = g.flow_qe()
+    s.flow_copy()
+    _, es_before = s.flow_qe()
+    s.flow_rk3(0.02, 1)
+    _, es_after = s.flow_qe()
+    print(f"E ratio after one step: {e_after / e_before:.6f} at 128^4, {es_after / es_before:.6f} at 16^4", flush=True)
+    assert abs(e_after / e_before - es_after / es_before) < 2e-3
+    s.close()
+    g.close()
