@@ -26,6 +26,10 @@ HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 HB_BYTES_PER_LINK = 162         # x sizeof(real): R+W U_mu(p), R U_mu(1-p), R 3 U_nu, per link-update
 WF_BYTES_PER_SITE = 720         # x sizeof(real): 3 fused RK3 stages with double-buffered W and one Z field
 METRIC = "HB link-updates/s & WF site-steps/s (FP32 vs FP64); flow-service p50/p99"
+# clover Q and E per site: 6 planes x 4 leaves x 3 SU(3) products of 216 flops,
+# plus the adjoint differences and the three pair traces
+CLOVER_FLOPS_PER_SITE = 15100
+VALU_PEAK_TFLOPS = 78.6         # MI355X FP64 / unpacked FP32 vector rate (packed FP32 is switched off)
 
 
 def log(msg):
@@ -72,6 +76,25 @@ def time_wf(qcb, ctx, L, prec, steps, warmup, eps=0.02, seed=11):
     achieved = WF_BYTES_PER_SITE / 3 * s * V / launch_s / 1e9
     return dict(site_steps_per_s=V * steps / wall, wall_s=wall, dev_ms=dev_ms,
                 avg_stage_launch_us=launch_s * 1e6, achieved_gbs=achieved)
+
+
+def time_clover(qcb, ctx, L, prec, reps=10, seed=11):
+    """clover Q, E of a lattice one heatbath sweep from a hot start: device time
+    of one measurement (k_clover, the fixed-order finalisation, the two-double
+    readback), averaged over reps"""
+    lat = qcb.Lattice(ctx, (L,) * 4, prec)
+    lat.hot(seed)
+    lat.hb_sweeps(6.0, 2, 1, seed=seed)
+    lat.clover_qe()
+    ctx.sync()
+    ctx.timing_begin()
+    for _ in range(reps):
+        q, e = lat.clover_qe()
+    ms = ctx.timing_end() / reps
+    V = lat.V
+    lat.close()
+    tflops = CLOVER_FLOPS_PER_SITE * V / (ms * 1e-3) / 1e12
+    return dict(ms=ms, tflops=tflops, frac_of_valu_peak=tflops / VALU_PEAK_TFLOPS, Q=q, E=e)
 
 
 def measured_traffic(kernel="k_hb_half<float"):
@@ -339,6 +362,12 @@ def main():
     wf32 = time_wf(qcb, ctx, L, qcb.F32, args.flow_steps, 2)
     log(f"[bench] WF FP32 {wf32['site_steps_per_s']:.3e} site-steps/s, {wf32['achieved_gbs']:.0f} GB/s")
     comp = {"hb_fp32": hb32, "wf_fp32": wf32}
+    cl = {"flops_per_site": CLOVER_FLOPS_PER_SITE, "valu_peak_tflops": VALU_PEAK_TFLOPS,
+          "fp32": time_clover(qcb, ctx, L, qcb.F32)}
+    if not args.no_fp64:
+        cl["fp64"] = time_clover(qcb, ctx, L, qcb.F64)
+    log(f"[bench] clover Q/E {L}^4: FP32 {cl['fp32']['ms']:.3f} ms ({cl['fp32']['tflops']:.1f} TFLOP/s)")
+    comp["clover_qe"] = cl
     if not args.no_fp64:
         hb64 = time_hb(qcb, ctx, L, qcb.F64, max(2, args.steps // 2), 1)
         log(f"[bench] HB FP64 {hb64['link_updates_per_s']:.3e} link-updates/s, {hb64['achieved_gbs']:.0f} GB/s")
