@@ -272,6 +272,48 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
     return QCB_OK;
 }
 
+// qcb_flow_steps for every lattice of the batch: each lattice's links are
+// copied into its flow fields, n_steps RK3 steps run with one launch per stage
+// for all lattices, and the flowed fields are copied back into the links; no
+// measurement.  The flowed field stays the lattice's flow state, as after
+// qcb_flow_steps.
+extern "C" int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps) {
+    if (!b || n_steps < 0) return QCB_EINVAL;
+    qcb_ctx* c = b->ctx;
+    int rc;
+    for (qcb_lattice* l : b->lats) {
+        if ((rc = ensure_flow_buffers(l))) return rc;
+        l->flow_start = b->gstep;
+        l->flow_active = true;
+    }
+    if ((rc = batch_upload(b))) return rc;
+    for (qcb_lattice* l : b->lats) {
+        Launch one = lat_launch(l, kFamLink);
+        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
+        else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
+    }
+    Launch gl = batch_launch(b, kFamLink);
+    for (int s = 0; s < n_steps; ++s) {
+        for (int st = 1; st <= 3; ++st) {
+            if (b->prec == QCB_F64) launch_flow_stage<double>(gl, st, b->gstep, eps);
+            else launch_flow_stage<float>(gl, st, b->gstep, eps);
+        }
+        b->gstep++;
+    }
+    for (qcb_lattice* l : b->lats) {
+        Launch one = lat_launch(l, kFamLink);
+        const void* cur = l->F[(b->gstep - l->flow_start) & 1u];
+        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)cur, (double*)l->U);
+        else launch_copy_field<float>(one, (const float*)cur, (float*)l->U);
+    }
+    QCB_HIP(c, hipGetLastError());
+    for (qcb_lattice* l : b->lats) {
+        l->gstep = b->gstep;
+        if ((rc = upload_desc(l))) return rc;   // single-lattice calls see the flow state
+    }
+    return QCB_OK;
+}
+
 extern "C" int qcb_batch_set_schedule(qcb_batch* b, const int order[8], uint32_t hb_grid) {
     if (!b || !order || hb_grid % 8) return QCB_EINVAL;
     int seen = 0;

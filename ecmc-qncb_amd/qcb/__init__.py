@@ -92,6 +92,7 @@ SIGNATURES = [
     ("qcb_batch_reunitarize", C.c_int, [_VP]),
     ("qcb_batch_plaquette", C.c_int, [_VP, _D]),
     ("qcb_batch_flow", C.c_int, [_VP, C.c_double, C.c_int, _D]),
+    ("qcb_batch_flow_steps", C.c_int, [_VP, C.c_double, C.c_int]),
     ("qcb_batch_set_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.c_uint32]),
     ("qcb_batch_get_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
     ("qcb_batch_autotune", C.c_int, [_VP, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
@@ -376,6 +377,10 @@ class Batch:
         out = np.zeros(self.n * (n_steps + 1) * 3)
         self._c(lib().qcb_batch_flow(self.h, eps, n_steps, _dp(out)))
         return out.reshape(self.n, n_steps + 1, 3)
+
+    def flow_steps(self, eps, n_steps):
+        """qcb_flow_steps on every lattice (no measurement)"""
+        self._c(lib().qcb_batch_flow_steps(self.h, eps, n_steps))
 
     def set_schedule(self, order, grid=0):
         self._c(lib().qcb_batch_set_schedule(self.h, (C.c_int * 8)(*order), grid))

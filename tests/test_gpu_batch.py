@@ -99,3 +99,22 @@ def test_autotune_keeps_links(ctx):
     assert sorted(order) == list(range(8)) and grid % 8 == 0
     for i in range(len(DIMS)):
         assert np.array_equal(b.lattice(i).get_links(), before[i])
+
+
+@pytest.mark.parametrize("prec", [qcb.F64, qcb.F32])
+def test_batch_flow_steps_equal_alone(ctx, prec):
+    """qcb_batch_flow_steps leaves every lattice's links bitwise as
+    qcb_flow_steps on that lattice alone, and its flow state measurable."""
+    b = qcb.Batch(ctx, DIMS, prec, BETAS, SEEDS)
+    b.hot()
+    b.hb_sweeps(2, 2)
+    start = [b.lattice(i).get_links() for i in range(len(DIMS))]
+    b.flow_steps(0.02, 3)
+    for i, d in enumerate(DIMS):
+        g = qcb.Lattice(ctx, d, prec)
+        g.set_links(start[i])
+        g.flow_steps(0.02, 3)
+        assert np.array_equal(b.lattice(i).get_links(), g.get_links())
+        assert b.lattice(i).flow_qe() == g.flow_qe()
+        g.close()
+    b.close()
