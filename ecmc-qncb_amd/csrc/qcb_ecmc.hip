@@ -265,15 +265,16 @@ __device__ inline U4 ctr_ev(uint32_t chain, unsigned long long n, uint32_t call,
 // random_SU3_epsilon (src/su3/utils.cpp:66-103): X01 X02 X12 near the identity.
 // X_f is the identity outside rows/columns (i, j), so M <- M X_f changes
 // columns i and j only; the sums are those of the full product with its
-// exact zero terms left out, in the same order.
-__device__ M3<double> random_su3_eps(double eps, const double r[9]) {
+// exact zero terms left out, in the same order.  a0 = sqrt(1 - eps^2), the
+// same for every factor, is formed once per launch.
+__device__ M3<double> random_su3_eps(double eps, double a0, const double r[9]) {
     M3<double> M;
     m3_identity(M);
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         const int i = f == 2 ? 1 : 0, j = f == 0 ? 1 : 2;
         const double n = sqrt(r[3 * f] * r[3 * f] + r[3 * f + 1] * r[3 * f + 1] + r[3 * f + 2] * r[3 * f + 2]);
-        const double a0 = sqrt(1 - eps * eps), a1 = eps * r[3 * f] / n, a2 = eps * r[3 * f + 1] / n,
+        const double a1 = eps * r[3 * f] / n, a2 = eps * r[3 * f + 1] / n,
                      a3 = eps * r[3 * f + 2] / n;
         // X_ii = a0 + i a3, X_jj = a0 - i a3, X_ij = a2 + i a1, X_ji = -a2 + i a1
 #pragma unroll
@@ -303,7 +304,8 @@ __device__ M3<double> random_su3_eps(double eps, const double r[9]) {
 
 // element i of the chain's matrix set of generation g: 0 the identity,
 // 2m+1 a near-identity matrix from {chain, g, 0xE5000000 | m, 0..2}, 2m+2 its adjoint
-__device__ M3<double> set_element(uint32_t chain, unsigned long long g, unsigned long long i, const EcmcArgs& a) {
+__device__ M3<double> set_element(uint32_t chain, unsigned long long g, unsigned long long i, const EcmcArgs& a,
+                                  double eps_a0) {
     M3<double> M;
     if (i == 0) {
         m3_identity(M);
@@ -318,7 +320,7 @@ __device__ M3<double> set_element(uint32_t chain, unsigned long long g, unsigned
         r[3 * k + 1] = u32d(o.y) - 0.5;
         r[3 * k + 2] = u32d(o.z) - 0.5;
     }
-    M = random_su3_eps(a.epsilon_set, r);
+    M = random_su3_eps(a.epsilon_set, eps_a0, r);
     return (i & 1) ? M : dag(M);
 }
 
@@ -420,12 +422,24 @@ __device__ M3<double> mul_cols01(const M3<double>& V, const M3<double>& R) {
     return o;
 }
 
+// orders LDS traffic between the lanes of one wavefront (a block is one
+// wavefront, so a team's lanes never span two)
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E, EcmcChainDev* __restrict__ chains,
                                                           EcmcArgs a) {
     const int tid = blockIdx.x * kThreads + threadIdx.x;
     const int cid = tid / kTeam, lane = tid % kTeam;
     if (cid >= a.n_chains) return;   // whole teams leave together
     const uint32_t chain = (uint32_t)cid;
+    // the event's proposed R' waits in LDS (one copy per team) from the start
+    // of the event to the lift, instead of holding 36 VGPRs through the staples
+    __shared__ double2 rn_lds[kThreads / kTeam][9];
+    double2* const my_rn = rn_lds[threadIdx.x / kTeam];
     EcmcChainDev st = chains[cid];
     int org[4], bc[4];   // block origin and block coordinates (block index = chain index)
     {
@@ -477,6 +491,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
     }
     unsigned long long gen = st.set_gen + 1;   // the set of this call
     const double beta_red = -(a.beta / 3.0);
+    const double eps_a0 = sqrt(1 - a.epsilon_set * a.epsilon_set);   // a0 of every near-identity factor
     double par_sample = 0.0;
     st.status = 0;
     for (unsigned long long guard = 0;; ++guard) {
@@ -501,7 +516,14 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
         // (it is used only if the event is a lift, where R is still this R)
         const U4 o2 = ctr_ev(chain, n, 2, a);
         const unsigned long long iset = __umul64hi(((unsigned long long)o2.x << 32) | o2.y, kSetSize);
-        const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a), R);
+        {
+            const M3<double> Rn = mul_nn(set_element(chain, gen, iset, a, eps_a0), R);
+            wave_sync();   // the previous event's reads are done
+            if (lane == 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) my_rn[k] = make_double2(Rn.v[2 * k], Rn.v[2 * k + 1]);
+            }
+        }
         // compute_reject_angles_fast (ecmc_mpi_cb.cpp:203-250)
         const M3<double> T = mul_dn(R, Ucur);
         // lane j < 6 solves staple j; the team takes the smallest angle, the
@@ -600,6 +622,14 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             const M3<double> U1 = ldl(E, a, bc, ls[1], ld[1]), U2 = ldl(E, a, bc, ls[2], ld[2]),
                              U3 = ldl(E, a, bc, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
+            wave_sync();
+            M3<double> Rn;
+#pragma unroll
+            for (int k = 0; k < 9; ++k) {
+                const double2 v = my_rn[k];
+                Rn.v[2 * k] = v.x;
+                Rn.v[2 * k + 1] = v.y;
+            }
             // lane i < 4 evaluates plaquette i against both R and R'; every
             // lane then holds all four dS and dS'
             double my_ds = 0.0, my_dsn = 0.0;
