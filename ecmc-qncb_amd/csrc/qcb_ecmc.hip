@@ -622,14 +622,19 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             const M3<double> U1 = ldl(E, a, bc, ls[1], ld[1]), U2 = ldl(E, a, bc, ls[2], ld[2]),
                              U3 = ldl(E, a, bc, ls[3], ld[3]);
             const bool fwd = (jmin & 1) == 0;
-            wave_sync();
-            M3<double> Rn;
+            // R' from LDS where it is used (each read behind its own fence, so
+            // that it is not hoisted back to the start of the lift)
+            auto load_rn = [&]() {
+                wave_sync();
+                M3<double> m;
 #pragma unroll
-            for (int k = 0; k < 9; ++k) {
-                const double2 v = my_rn[k];
-                Rn.v[2 * k] = v.x;
-                Rn.v[2 * k + 1] = v.y;
-            }
+                for (int k = 0; k < 9; ++k) {
+                    const double2 v = my_rn[k];
+                    m.v[2 * k] = v.x;
+                    m.v[2 * k + 1] = v.y;
+                }
+                return m;
+            };
             // lane i < 4 evaluates plaquette i against both R and R'; every
             // lane then holds all four dS and dS'
             double my_ds = 0.0, my_dsn = 0.0;
@@ -649,7 +654,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
                     V.v[k] = vk == 1 ? U0.v[k] : (vk == 2 ? U3.v[k] : (vk == 3 ? U3d.v[k] : (vk == 4 ? Vb.v[k] : id)));
                 }
                 my_ds = frozen_near(a, si, di) ? 0.0 : sg * compute_ds(M, mul_cols01(V, R));
-                my_dsn = sg * compute_ds(M, mul_cols01(V, Rn));
+                my_dsn = sg * compute_ds(M, mul_cols01(V, load_rn()));
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
             double absd[4], sum = 0.0;
@@ -670,7 +675,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             const double dSn = __shfl(my_dsn, base + idx, 64);
             double neps = -sgn[idx];
             if (fabs(dS) < fabs(dSn) || u32d(o1.w) < fabs(dSn) / fabs(dS)) {
-                R = Rn;
+                R = load_rn();
                 neps = -dsgn(dSn);
             }
             const Site xn = idx == 0 ? ls[0] : idx == 1 ? ls[1] : idx == 2 ? ls[2] : ls[3];
