@@ -353,7 +353,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=60)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+        print(f"bench.py measures one GPU: qcb has no multi-GPU path (asked for --gpus {args.gpus}); "
               "independent lattices shard one-or-more per GPU with no communication", file=sys.stderr)
         sys.exit(2)
 

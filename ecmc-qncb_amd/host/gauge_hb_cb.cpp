@@ -6,7 +6,7 @@
 // through the qcb C-ABI.  The whole n_core_dims^4 * L_core^4 lattice lives on
 // the one device, so there are no ranks, no frozen links and no halos; the
 // random shifts are kept (same host mt19937_64 draws, device translate).
-// Extra input keys of this build: precision = fp64 | fp32, reunit_every = n.
+// Extra input keys of this driver: precision = fp64 | fp32, reunit_every = n.
 #include <chrono>
 #include <fstream>
 #include <filesystem>

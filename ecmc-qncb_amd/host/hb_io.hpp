@@ -35,7 +35,7 @@ struct RunParamsHbCB {
     std::string run_name = "c";
     std::string run_dir = "data";
     int save_each_shifts = 2;
-    // keys of this build (ignored by the reference parser, so inputs stay
+    // keys of this driver (ignored by the reference parser, so inputs stay
     // interchangeable): link precision and reunitarisation cadence
     std::string precision = "fp64";   // "fp64" | "fp32"
     int reunit_every = -1;            // sweeps between reunitarisations; -1: 4 for fp32, never for fp64
