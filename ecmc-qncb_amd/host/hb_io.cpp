@@ -73,8 +73,18 @@ void load_params(const std::string& filename, RunParamsHbCB& rp) {
     if (has("run_name")) rp.run_name = cfg["run_name"];
     if (has("run_dir")) rp.run_dir = cfg["run_dir"];
     if (has("save_each_shifts")) rp.save_each_shifts = std::stoi(cfg["save_each_shifts"]);
-    if (has("precision")) rp.precision = cfg["precision"];
-    if (has("reunit_every")) rp.reunit_every = std::stoi(cfg["reunit_every"]);
+    // keys of this driver only: checked strictly, since the reference never reads them
+    if (has("precision")) {
+        rp.precision = cfg["precision"];
+        if (rp.precision != "fp64" && rp.precision != "fp32")
+            throw std::runtime_error("precision must be fp64 or fp32, not '" + rp.precision + "'");
+    }
+    if (has("reunit_every")) {
+        size_t used = 0;
+        rp.reunit_every = std::stoi(cfg["reunit_every"], &used);
+        if (used != cfg["reunit_every"].size() || rp.reunit_every < 0)
+            throw std::runtime_error("reunit_every must be a count >= 0");
+    }
 }
 
 bool read_params(RunParamsHbCB& params, const std::string& input) {

@@ -84,3 +84,15 @@ def test_defaults_and_cold_start_literal(tmp_path):
 def test_missing_file(tmp_path):
     r = subprocess.run([DRIVER, "--print-params", str(tmp_path / "nope.txt")], capture_output=True, text=True)
     assert r.returncode == 1 and "Can't open file" in r.stderr
+
+
+def test_precision_keys(tmp_path):
+    """The driver's own keys: precision = fp64 | fp32 (fp32 reunitarises every
+    4 sweeps unless reunit_every says otherwise); other values are refused."""
+    r = run(tmp_path, "precision = fp32\n")
+    assert r.returncode == 0 and "precision=fp32 reunit_every=-1" in r.stdout
+    r = run(tmp_path, "precision = fp32\nreunit_every = 10\n")
+    assert r.returncode == 0 and "precision=fp32 reunit_every=10" in r.stdout
+    for bad in ("precision = fp16\n", "reunit_every = -2\n", "reunit_every = 3x\n", "reunit_every = four\n"):
+        r = run(tmp_path, bad)
+        assert r.returncode == 1 and "Error reading input" in r.stderr, bad

@@ -129,3 +129,50 @@ def test_driver_run_and_resume_match_replay(tmp_path):
             params = f.read()
         assert params.count("Saved final state !") == run + 1
         assert os.path.exists(os.path.join(run_dir, "e2e", "e2e_seed", "e2e_seed0.txt"))
+
+
+def test_fp32_driver_tracks_fp64(tmp_path):
+    """precision = fp32 (two-row links, reunitarised every 4 sweeps by
+    default): the driver draws the same shifts and Philox keys from the host
+    stream as in FP64, so its heatbath consumes identical uniforms and its
+    plaquettes track the FP64 run's (two independent 8^4 chains differ by
+    ~1e-3); the FP32 ILDG checkpoint holds SU(3) links."""
+    outs = {}
+    for prec in ("fp64", "fp32"):
+        run_dir = tmp_path / prec
+        inp = tmp_path / f"{prec}.txt"
+        inp.write_text(f"""L_core = 8
+n_core_dims = 1
+cold_start = true
+seed = 4321
+N_shift_therm = 0
+N_shift = 6
+beta = 6.0
+N_sweeps = 1
+N_hits = 2
+N_shift_plaquette = 1
+topo = false
+run_name = p
+run_dir = {run_dir}
+save_each_shifts = 1000
+precision = {prec}
+""")
+        out = run_driver(str(inp), str(tmp_path / f"{prec}.log"))
+        shifts = re.findall(r"Shift \((\d+), (\d+), (\d+), (\d+)\)", out)
+        plaq = np.loadtxt(os.path.join(run_dir, "p", "p_plaquette.txt"))
+        outs[prec] = (shifts, plaq, os.path.join(run_dir, "p", "p"))
+    s64, p64, _ = outs["fp64"]
+    s32, p32, ildg32 = outs["fp32"]
+    assert s32 == s64 and len(s64) == 6
+    assert p32.shape == p64.shape == (6,)
+    # measured: |dP| ~1e-7 on all six shifts
+    assert np.abs(p32 - p64).max() < 1e-5
+    assert np.all(p32 > 0.2) and np.all(p32 < 0.8)
+    ctx = qcb.Context(0)
+    g = qcb.Lattice(ctx, (8, 8, 8, 8))
+    g.ildg_load(ildg32)
+    U = g.get_links().reshape(-1, 3, 3)
+    assert np.abs(U @ U.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-5
+    assert abs(g.plaquette() - p32[-1]) < 1e-5
+    g.close()
+    ctx.close()
