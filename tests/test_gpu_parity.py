@@ -195,9 +195,9 @@ def test_heatbath_injected_from_hot_start_compounded(ctx):
     assert abs(g.plaquette() - o.plaquette()) < 1e-12
 
 
-def test_heatbath_philox_matches_oracle(ctx):
+@pytest.mark.parametrize("dims", [(4, 4, 6, 4), (10, 6, 4, 8)])
+def test_heatbath_philox_matches_oracle(ctx, dims):
     """Device Philox uniforms = the oracle's counter scheme: per-sweep links to 1e-12."""
-    dims = (4, 4, 6, 4)
     g, o = hot_pair(ctx, dims, 9)
     for s in range(2):
         o.sweep_philox(5.9, 2, 123, 100 + s)
@@ -259,8 +259,8 @@ def test_reunitarize_and_translate(ctx):
     assert np.array_equal(g.get_links(), o.links)
 
 
-def test_flow_matches_oracle(ctx):
-    dims = (6, 4, 4, 6)
+@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (10, 6, 4, 8)])
+def test_flow_matches_oracle(ctx, dims):
     g, o = hot_pair(ctx, dims, 17)
     for s in range(2):
         o.sweep_philox(6.0, 1, 8, s)
