@@ -635,11 +635,12 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
                 }
                 return m;
             };
-            // lane i < 4 evaluates plaquette i against both R and R'; every
-            // lane then holds all four dS and dS'
-            double my_ds = 0.0, my_dsn = 0.0;
-            if (lane < 4) {
-                const int i = lane;
+            // lane i < 4 evaluates plaquette i against R, lane 4 + i against R';
+            // every lane then reads all four dS and the chosen dS'
+            double my_ds = 0.0;
+            if (lane < 8) {
+                const int i = lane & 3;
+                const bool prime = lane >= 4;
                 const Site si = i == 0 ? ls[0] : i == 1 ? ls[1] : i == 2 ? ls[2] : ls[3];
                 const int di = i == 0 ? ld[0] : i == 1 ? ld[1] : i == 2 ? ld[2] : ld[3];
                 const M3<double> M = plaq_loop(fwd, U0, U1, U2, U3);
@@ -653,8 +654,12 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
                     const double id = (k == 0 || k == 8 || k == 16) ? 1.0 : 0.0;   // re of the diagonal
                     V.v[k] = vk == 1 ? U0.v[k] : (vk == 2 ? U3.v[k] : (vk == 3 ? U3d.v[k] : (vk == 4 ? Vb.v[k] : id)));
                 }
-                my_ds = frozen_near(a, si, di) ? 0.0 : sg * compute_ds(M, mul_cols01(V, R));
-                my_dsn = sg * compute_ds(M, mul_cols01(V, load_rn()));
+                const M3<double> Rp = load_rn();
+                M3<double> Rs;
+#pragma unroll
+                for (int k = 0; k < 18; ++k) Rs.v[k] = prime ? Rp.v[k] : R.v[k];
+                const double d = sg * compute_ds(M, mul_cols01(V, Rs));
+                my_ds = (!prime && frozen_near(a, si, di)) ? 0.0 : d;
             }
             const int base = (threadIdx.x / kTeam) * kTeam;
             double absd[4], sum = 0.0;
@@ -672,7 +677,7 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
             const double u = u32d(o1.z);   // selectVariable (ecmc_mpi_cb.cpp:252-260)
             const int idx = u < prob[0] ? 0 : (u < prob[0] + prob[1] ? 1 : (u < prob[0] + prob[1] + prob[2] ? 2 : 3));
             const double dS = sgn[idx] * absd[idx];
-            const double dSn = __shfl(my_dsn, base + idx, 64);
+            const double dSn = __shfl(my_ds, base + 4 + idx, 64);
             double neps = -sgn[idx];
             if (fabs(dS) < fabs(dSn) || u32d(o1.w) < fabs(dSn) / fabs(dS)) {
                 R = load_rn();
