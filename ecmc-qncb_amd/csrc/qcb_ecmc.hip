@@ -48,7 +48,8 @@ struct EcmcArgs {
 
 constexpr uint64_t kSetSize = 10001;   // ecmc_set (src/su3/utils.cpp:119-127), 1 + 5000 pairs
 // A chain is worked by a team of 16 lanes: lanes 0-5 form and solve the six
-// staples of an event in parallel, lanes 0-3 the four plaquettes of a lift;
+// staples of an event in parallel, lanes 0-7 the four plaquettes of a lift
+// against R and R';
 // the rest of the event is computed identically by every lane of the team.
 // A wavefront's teams run one instruction stream, so more chains per
 // wavefront share its instructions, while fewer leave less to run one after
