@@ -175,7 +175,11 @@ class Lattice:
 
     @property
     def raw(self) -> np.ndarray:
-        return np.ctypeslib.as_array(self.s.U, shape=(self.V * 72,))
+        # the array's base keeps this Lattice alive, so a view taken from a
+        # temporary (e.g. f().links.copy()) never outlives the C buffer
+        buf = (C.c_double * (self.V * 72)).from_address(C.addressof(self.s.U.contents))
+        buf._owner = self
+        return np.ctypeslib.as_array(buf)
 
     @property
     def links(self) -> np.ndarray:
