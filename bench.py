@@ -303,6 +303,50 @@ def precision_report(qcb, ctx, L=32, sweeps=500, flow_steps=150, seed=11, beta=6
                 equilibrium=equilibrium, plaquette_measurement=p_meas, flow=rows)
 
 
+def ecmc_report(qcb, ctx, L=32, hb_sweeps=3000, samples=1500, seed=29):
+    """Equilibrium mean plaquette of the device event chains (4^4 blocks, a
+    random shift before every sample) against the device heatbath, both FP64
+    at beta = 6.0 on an L^4 lattice; binned errors (20 bins after 10% burn-in)."""
+    import numpy as np
+
+    def binned(x, nb=20):
+        x = np.asarray(x)
+        n = len(x) // nb
+        m = x[:n * nb].reshape(nb, n).mean(axis=1)
+        return float(m.mean()), float(m.std(ddof=1) / np.sqrt(nb))
+
+    dims = (L,) * 4
+    h = qcb.Lattice(ctx, dims)
+    h.hot(seed)
+    h.hb_sweeps(6.0, 2, 200, seed=seed)
+    ph = []
+    for s in range(hb_sweeps):
+        h.hb_sweeps(6.0, 2, 1, seed=seed, sweep_idx=200 + s)
+        ph.append(h.plaquette())
+    g = qcb.Lattice(ctx, dims)
+    g.set_links(h.get_links())
+    h.close()
+    e = qcb.Ecmc(g, qcb.ecmc_params(beta=6.0, theta_sample=1000.0, theta_refresh_site=430.0,
+                                    theta_refresh_R=51.0, seed=17, block=(4, 4, 4, 4), max_events=10 ** 7))
+    rng = np.random.default_rng(19)
+    pe = []
+    for s in range(samples):
+        g.translate(rng.integers(0, L, size=4))
+        e.sample(1)
+        pe.append(g.plaquette())
+        if s % 250 == 0:
+            log(f"[ecmc-report] sample {s}")
+    e.chains()   # raises if any chain stopped early
+    e.close()
+    g.close()
+    m_hb, e_hb = binned(ph[hb_sweeps // 10:])
+    m_ec, e_ec = binned(pe[samples // 10:])
+    sigma = float(np.hypot(e_hb, e_ec))
+    return dict(lattice=list(dims), beta=6.0, heatbath=dict(sweeps=hb_sweeps, P=m_hb, err=e_hb),
+                ecmc=dict(samples=samples, chains=(L // 4) ** 4, P=m_ec, err=e_ec),
+                diff=m_ec - m_hb, sigma=sigma, n_sigma=abs(m_ec - m_hb) / sigma)
+
+
 def wf_cpu_baseline(L=16, steps=2):
     """The reference's RK3 Wilson flow step (FP64, src/flow/gradient_flow.cpp:15-128)
     restated by the oracle, one core, on a hot L^4 lattice: site-steps/s."""
@@ -351,6 +395,8 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="run the FP32-vs-FP64 comparison (32^4, 500 sweeps, 150 flow steps) and print it")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--ecmc-report", action="store_true",
+                    help="compare the event chains' equilibrium plaquette with the heatbath's (32^4, FP64) and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: qcb has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -362,6 +408,10 @@ def main():
     L = args.L
     if args.precision_report:
         print(json.dumps(precision_report(qcb, ctx, L)))
+        ctx.close()
+        return
+    if args.ecmc_report:
+        print(json.dumps(ecmc_report(qcb, ctx, L)))
         ctx.close()
         return
     log(f"[bench] HB FP32 {L}^4: warmup {args.warmup}, timed {args.steps} sweeps")
