@@ -24,7 +24,13 @@ def ctx():
 
 def test_largest_fp32_lattice(ctx):
     print(f"max-size lattice {BIG} FP32", flush=True)
-    g = qcb.Lattice(ctx, BIG, qcb.F32)
+    try:
+        g = qcb.Lattice(ctx, BIG, qcb.F32)
+        g.flow_copy()   # allocates the flow buffers (~200 GB in all) before anything else
+    except qcb.QcbError as err:
+        if err.code == qcb.ENOMEM:
+            pytest.skip(f"the device cannot hold a {BIG} FP32 lattice with its flow buffers: {err}")
+        raise
     assert g.plaquette() == pytest.approx(1.0, abs=1e-12)
     q, e = g.clover_qe()
     assert abs(q) < 1e-9 and abs(e) < 1e-12
