@@ -69,10 +69,11 @@ typedef struct {
 int  qo_lat_init(qo_lat* l, const int L[4]);
 void qo_lat_free(qo_lat* l);
 void qo_lat_cold(qo_lat* l);                                            /* GaugeField.cpp:23-33 */
-void qo_lat_hot_ref(qo_lat* l, qo_mt64* g);                             /* GaugeField.cpp:10-20 + utils.cpp:13-30; Householder conventions parity unpinned */
+void qo_lat_hot_ref(qo_lat* l, qo_mt64* g);                             /* GaugeField.cpp:10-20 + utils.cpp:13-30 (clang/IntelLLVM draw order) */
 void qo_lat_hot_philox(qo_lat* l, uint64_t seed);                       /* device hot start restated: Box-Muller + Gram-Schmidt + det^(1/3) */
 void qo_haar_su3_philox(const uint32_t key[2], uint32_t c0, uint32_t c1, uint32_t c2, double* Q); /* the same on counters {c0,c1,c2,0..8} */
-void qo_random_su3_ref(qo_mt64* g, double* Q);                          /* utils.cpp:13-30 (one matrix); Householder conventions parity unpinned */
+void qo_random_su3_ref(qo_mt64* g, double* Q);                          /* utils.cpp:13-30 (one matrix), clang/IntelLLVM draw order */
+void qo_random_su3_ref_order(qo_mt64* g, double* Q, int imag_first);    /* utils.cpp:13-30 with the draw order of a clang (0) or GCC (1) build */
 void qo_staple(const qo_lat* l, size_t site, int mu, double* out);      /* GaugeField.cpp:74-97 */
 double qo_plaquette(const qo_lat* l);                                   /* observables_mpi.cpp:8-43 */
 void qo_clover_F(const qo_lat* l, size_t site, int mu, int nu, double* F); /* observables_mpi.cpp:61-85 */
@@ -98,6 +99,9 @@ long qo_hb_sweep_ref(qo_lat* l, double beta, int n_hits, qo_mt64* g);
  * with hs = 2*mu + parity.  Returns the number of exhausted (link,hit,subgroup)
  * budgets; those leave the link unchanged for that subgroup. */
 long qo_hb_sweep_cb_injected(qo_lat* l, double beta, int n_hits, const double* u, int K);
+/* One CM hit with injected uniforms u[g*(4K+2) + slot] (heatbath_mpi.cpp:11-34);
+ * returns the number of subgroups whose K attempts all rejected. */
+int qo_cm_hit_injected(double* U, const double* A, double beta, const double* u, int K);
 
 /* Checkerboard order with Philox4x32-10 uniforms (counter scheme documented
  * in qo_rng.c and in the device header). */

@@ -67,16 +67,26 @@ void qo_lat_cold(qo_lat* l) {
 
 /* Householder QR in the form Eigen's HouseholderQR uses, Q = H0 H1 H2 with
  * H_k = 1 - conj(tau_k) v v^dag, then Q / det(Q)^(1/3) on the principal branch
- * (src/su3/utils.cpp:13-30).  Draw order re-then-im per entry is a choice:
- * the reference's Complex(gauss(rng), gauss(rng)) leaves it to the compiler.
- * Parity unpinned (Eigen absent). */
-void qo_random_su3_ref(qo_mt64* g, double* Q) {
+ * (src/su3/utils.cpp:13-30).
+ * Draw order: the reference's Complex(gauss(rng), gauss(rng)) leaves the order
+ * of the two draws to the compiler.  imag_first = 0 is a clang-based build
+ * (IntelLLVM, the reference cluster's compiler, modules_load.sh:5): arguments
+ * left to right, the first draw is the real part.  imag_first = 1 is a GCC
+ * build (CMakeLists.txt:97-102 branch): right to left, the first draw is the
+ * imaginary part.  Both orders are pinned by compiled probes, the QR
+ * conventions by QR uniqueness plus the geometric form of each reflection
+ * (tests/test_oracle_su3.py). */
+void qo_random_su3_ref_order(qo_mt64* g, double* Q, int imag_first) {
     qo_c A[3][3];
     /* a fresh normal_distribution per call: 18 draws are 9 whole polar pairs */
     double gs[18];
     for (int k = 0; k < 18; k += 2) qo_mt64_normal_pair(g, &gs[k], &gs[k + 1]);
+    const int a = imag_first ? 1 : 0;
     for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) { A[i][j].re = gs[2 * (3 * i + j)]; A[i][j].im = gs[2 * (3 * i + j) + 1]; }
+        for (int j = 0; j < 3; ++j) {
+            A[i][j].re = gs[2 * (3 * i + j) + a];
+            A[i][j].im = gs[2 * (3 * i + j) + 1 - a];
+        }
     qo_c v[3][3]; /* v[k][0..] householder vectors, v[k][k] = 1 */
     qo_c tau[3];
     for (int k = 0; k < 3; ++k) {
@@ -161,6 +171,9 @@ void qo_random_su3_ref(qo_mt64* g, double* Q) {
         Q[2 * k + 1] = im;
     }
 }
+
+/* the clang / IntelLLVM draw order (the reference cluster's build) */
+void qo_random_su3_ref(qo_mt64* g, double* Q) { qo_random_su3_ref_order(g, Q, 0); }
 
 void qo_lat_hot_ref(qo_lat* l, qo_mt64* g) {
     /* interior sites in t,z,y,x order, mu inner (src/gauge/GaugeField.cpp:10-20) */
@@ -526,6 +539,17 @@ static int kp_inj_cb(void* vctx, int g, double beta, double k, double out[4]) {
         return -1;
     }
     return 0;
+}
+
+/* One Cabibbo-Marinari hit (src/heatbath/heatbath_mpi.cpp:11-34) on U with
+ * staple A and injected uniforms u[g*(4K+2) + slot] for subgroup g (the
+ * layout of the device probe QCB_PROBE_CM_HIT).  A subgroup whose K
+ * attempts all reject is skipped; returns how many were. */
+int qo_cm_hit_injected(double* U, const double* A, double beta, const double* u, int K) {
+    if (K < 1 || K > 64) return -1;
+    inj_ctx ctx = {u, 1, K, 0, 0, 1, 0, 0};
+    cm_hit(U, A, beta, kp_inj_cb, &ctx);
+    return (int)ctx.exhausted;
 }
 
 long qo_hb_sweep_cb_injected(qo_lat* l, double beta, int n_hits, const double* u, int K) {

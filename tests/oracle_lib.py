@@ -93,6 +93,7 @@ def lib():
             "qo_lat_free": (None, [P(QoLat)]),
             "qo_lat_cold": (None, [P(QoLat)]),
             "qo_random_su3_ref": (None, [P(QoMt64), P(C.c_double)]),
+            "qo_random_su3_ref_order": (None, [P(QoMt64), P(C.c_double), C.c_int]),
             "qo_lat_hot_ref": (None, [P(QoLat), P(QoMt64)]),
             "qo_lat_hot_philox": (None, [P(QoLat), C.c_uint64]),
             "qo_hot_link_philox": (None, [C.c_uint64, C.c_uint32, C.c_uint32, _D]),
@@ -108,6 +109,7 @@ def lib():
             "qo_topo_charge_flowed": (None, [P(QoLat), C.c_double, C.c_int, C.c_int, _D]),
             "qo_hb_sweep_ref": (C.c_long, [P(QoLat), C.c_double, C.c_int, P(QoMt64)]),
             "qo_hb_sweep_cb_injected": (C.c_long, [P(QoLat), C.c_double, C.c_int, _D, C.c_int]),
+            "qo_cm_hit_injected": (C.c_int, [_D, _D, C.c_double, _D, C.c_int]),
             "qo_hb_sweep_cb_philox": (None, [P(QoLat), C.c_double, C.c_int, C.c_uint64, C.c_uint32]),
             "qo_ref_run": (C.c_int, [P(RunParams), _D, _D, P(C.c_int), _D, P(RunStats)]),
             "qo_now": (C.c_double, []),
@@ -262,10 +264,12 @@ class Mt64:
     def next(self) -> int:
         return lib().qo_mt64_next(C.byref(self.s))
 
-    def random_su3(self) -> np.ndarray:
-        """one SU(3) draw of the reference's hot start (src/su3/utils.cpp:13-30)"""
+    def random_su3(self, imag_first: bool = False) -> np.ndarray:
+        """one SU(3) draw of the reference's hot start (src/su3/utils.cpp:13-30);
+        imag_first selects a GCC build's draw order (default: clang/IntelLLVM)"""
         out = np.zeros(18)
-        lib().qo_random_su3_ref(C.byref(self.s), out.ctypes.data_as(C.POINTER(C.c_double)))
+        lib().qo_random_su3_ref_order(C.byref(self.s), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                      int(imag_first))
         return out.view(np.complex128).reshape(3, 3)
 
     def canonical(self) -> float:

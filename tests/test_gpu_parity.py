@@ -123,11 +123,16 @@ def test_probe_exp_proj_reunit(ctx):
         assert np.abs(qo.c3(x) - b).max() < 1e-14
 
 
-def test_probe_kp_and_cm_hit(ctx):
-    rng = np.random.default_rng(2)
-    K = 6
-    n = 512
-    inp = np.concatenate([rng.uniform(0.05, 3.0, size=(n, 1)), rng.uniform(size=(n, 4 * K + 2))], 1)
+def _kp_inputs(seed, n, K):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([rng.uniform(0.05, 3.0, size=(n, 1)), rng.uniform(size=(n, 4 * K + 2))], 1)
+
+
+def test_probe_kp_fp64(ctx):
+    """Kennedy-Pendleton draws on injected uniforms: accepted attempt index
+    exact and x to 1e-14 against the oracle (heatbath.cpp:8-36)."""
+    K, n = 6, 512
+    inp = _kp_inputs(2, n, K)
     out = ctx.probe(qcb.F64, qcb.PROBE_KP, inp, 6.0, K)
     for row, o in zip(inp, out):
         x = np.zeros(4)
@@ -135,6 +140,78 @@ def test_probe_kp_and_cm_hit(ctx):
         assert a == int(o[0])
         if a >= 0:
             assert np.abs(o[1:] - x).max() < 1e-14
+
+
+def test_probe_kp_fp32(ctx):
+    """The float Kennedy-Pendleton draw (hardware log/cos/sin/rcp) against
+    the FP64 oracle.  Items whose accept/reject decisions are within 1e-5 of
+    the boundary delta^2 = 1 - lambda^2 at some attempt up to the accepted one
+    are float-sensitive by construction and excluded (at most 1% of them).
+    For the rest the accepted attempt is the oracle's and
+    |dx0| < 4e-6, |dx_c| < 1e-5 + 4e-7 / r with r = sqrt(1 - a0^2): a0 = 1 -
+    2 lambda^2 carries the float rounding of the logarithms (lambda^2 <= 1
+    when accepted), and 1 - a0^2 loses ~1.2e-7 absolute to cancellation,
+    which the square root divides by 2r."""
+    import cmhit_math as cm
+    K, n = 6, 4096
+    inp = _kp_inputs(12, n, K)
+    out = ctx.probe(qcb.F32, qcb.PROBE_KP, inp, 6.0, K)
+    skipped = 0
+    for row, o in zip(inp, out):
+        u = np.ascontiguousarray(row[1:])
+        x = np.zeros(4)
+        a = qo.lib().qo_kp_injected(6.0, row[0], qo.dp(u), K, qo.dp(x))
+        margins, rs = cm.kp_margins(6.0, float(np.float32(row[0])), u, K)
+        if min(abs(m) for m in margins) < 1e-5:
+            skipped += 1
+            continue
+        assert a == int(o[0]), (row[0], margins)
+        if a >= 0:
+            assert abs(o[1] - x[0]) < 4e-6
+            assert np.abs(o[2:] - x[1:]).max() < 1e-5 + 4e-7 / max(rs, 1e-12)
+    assert skipped <= n // 100
+
+
+def _cm_inputs(seed, n, K):
+    import cmhit_math as cm
+    r = np.random.default_rng(seed)
+    nslot = 4 * K + 2
+    U = [cm.random_su3(r) for _ in range(n)]
+    A = [cm.staple_like(r) for _ in range(n)]
+    u = r.uniform(size=(n, 3 * nslot))
+    for t in range(0, n, 16):                        # some subgroups exhaust their budget
+        g = (t // 16) % 3
+        u[t, g * nslot:g * nslot + 4 * K:4] = 1.0 - 1e-12
+    inp = np.concatenate([np.stack([qo.m3(x) for x in U]), np.stack([qo.m3(x) for x in A]), u], 1)
+    return U, A, u, inp
+
+
+def test_probe_cm_hit(ctx):
+    """One Cabibbo-Marinari hit (heatbath_mpi.cpp:11-34) on injected uniforms:
+    FP64 against the oracle's cm_hit to 1e-14; FP32 against it within float
+    rounding.  The FP32 check excludes float-sensitive items as the KP test
+    does (any attempt of any subgroup within 1e-5 of its boundary, at most 2%
+    of items) and bounds the rest by 2e-5 + 6e-7 / r_min, r_min the smallest
+    sphere radius sqrt(1 - a0^2) of the three draws."""
+    import cmhit_math as cm
+    K, n = 6, 512
+    U, A, u, inp = _cm_inputs(21, n, K)
+    out64 = ctx.probe(qcb.F64, qcb.PROBE_CM_HIT, inp, 6.0, K).view(np.complex128).reshape(n, 3, 3)
+    out32 = ctx.probe(qcb.F32, qcb.PROBE_CM_HIT, inp, 6.0, K).view(np.complex128).reshape(n, 3, 3)
+    skipped = 0
+    for t in range(n):
+        want = qo.m3(U[t].copy())
+        qo.lib().qo_cm_hit_injected(want, qo.dp(qo.m3(A[t])), 6.0, qo.dp(np.ascontiguousarray(u[t])), K)
+        want = qo.c3(want)
+        assert np.abs(out64[t] - want).max() < 1e-14
+        _, info = cm.cm_hit(U[t], A[t], 6.0, u[t], K)
+        if min(min(abs(m) for m in margins) for _, _, margins, _ in info) < 1e-5:
+            skipped += 1
+            continue
+        rmin = min([rs for _, x, _, rs in info if x is not None] or [1.0])
+        assert np.abs(out32[t] - want).max() < 2e-5 + 6e-7 / max(rmin, 1e-12)
+        assert np.abs(out32[t] @ out32[t].conj().T - np.eye(3)).max() < 2e-6
+    assert skipped <= n // 50
 
 
 def test_probe_uniform_words(ctx):
