@@ -14,6 +14,31 @@
 
 using namespace qcb;
 
+namespace {
+// A stream-ordered device temporary, released on every return path: the free
+// is queued behind whatever work was enqueued on the stream before it.
+struct DevTmp {
+    qcb_ctx* c;
+    void* p = nullptr;
+    explicit DevTmp(qcb_ctx* ctx) : c(ctx) {}
+    DevTmp(const DevTmp&) = delete;
+    DevTmp& operator=(const DevTmp&) = delete;
+    ~DevTmp() {
+        if (p) (void)hipFreeAsync(p, c->stream);
+    }
+    template <class T>
+    int alloc(size_t bytes, T** out) {
+        hipError_t e = hipMallocAsync(&p, bytes, c->stream);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return qcb_set_error(c, QCB_ENOMEM, "cannot allocate a %zu-byte temporary: %s", bytes, hipGetErrorString(e));
+        }
+        *out = static_cast<T*>(p);
+        return QCB_OK;
+    }
+};
+}  // namespace
+
 // ------------------------------------------------------------ context -----
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...) {
@@ -74,6 +99,14 @@ extern "C" int qcb_ctx_synchronize(qcb_ctx* ctx) {
     return QCB_OK;
 }
 
+extern "C" int qcb_ctx_set_kp_budget(qcb_ctx* ctx, uint32_t max_attempts) {
+    if (!ctx) return QCB_EINVAL;
+    if (max_attempts < 1 || max_attempts > (1u << 20))
+        return qcb_set_error(ctx, QCB_EINVAL, "Kennedy-Pendleton attempt cap must be in 1..2^20");
+    ctx->kp_cap = max_attempts;
+    return QCB_OK;
+}
+
 extern "C" int qcb_timing_begin(qcb_ctx* ctx) {
     if (!ctx) return QCB_EINVAL;
     QCB_HIP(ctx, hipEventRecord(ctx->ev0, ctx->stream));
@@ -130,6 +163,8 @@ extern "C" int qcb_lattice_create(qcb_ctx* ctx, const int dims[4], int prec, qcb
 
 extern "C" int qcb_lattice_destroy(qcb_lattice* lat) {
     if (!lat) return QCB_EINVAL;
+    if (lat->n_users > 0)
+        return qcb_set_error(lat->ctx, QCB_ESTATE, "%d event-chain handles still use this lattice", lat->n_users);
     (void)hipSetDevice(lat->ctx->device);
     (void)hipStreamSynchronize(lat->ctx->stream);
     lat->ctx->n_live--;
@@ -164,13 +199,13 @@ extern "C" int qcb_lattice_set_links(qcb_lattice* lat, const double* aos) {
     qcb_ctx* ctx = lat->ctx;
     size_t n = (size_t)lat->V * 72;
     double* d = nullptr;
-    QCB_HIP(ctx, hipMallocAsync((void**)&d, n * sizeof(double), ctx->stream));
+    DevTmp tmp(ctx);
+    if (int rc = tmp.alloc(n * sizeof(double), &d)) return rc;
     QCB_HIP(ctx, hipMemcpyAsync(d, aos, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     Launch g = lat_launch(lat, kFamLink);
     if (lat->prec == QCB_F64) launch_aos_to_soa<double>(g, d, (double*)lat->U);
     else launch_aos_to_soa<float>(g, d, (float*)lat->U);
     QCB_HIP(ctx, hipGetLastError());
-    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QCB_OK;
 }
@@ -179,13 +214,13 @@ static int download_field(qcb_lattice* lat, const void* field, double* aos) {
     qcb_ctx* ctx = lat->ctx;
     size_t n = (size_t)lat->V * 72;
     double* d = nullptr;
-    QCB_HIP(ctx, hipMallocAsync((void**)&d, n * sizeof(double), ctx->stream));
+    DevTmp tmp(ctx);
+    if (int rc = tmp.alloc(n * sizeof(double), &d)) return rc;
     Launch g = lat_launch(lat, kFamLink);
     if (lat->prec == QCB_F64) launch_soa_to_aos<double>(g, (const double*)field, d);
     else launch_soa_to_aos<float>(g, (const float*)field, d);
     QCB_HIP(ctx, hipGetLastError());
     QCB_HIP(ctx, hipMemcpyAsync(aos, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QCB_OK;
 }
@@ -216,15 +251,14 @@ extern "C" int qcb_hot(qcb_lattice* lat, uint64_t seed) {
 
 // ---------------------------------------------------------- observables ---
 
-int enqueue_plaquette(qcb_lattice* lat, double* d_out) {
+int enqueue_plaquette(qcb_lattice* lat, int which, double* d_out) {
     Launch g = lat_launch(lat, kFamSite);
-    Launch f = lat_launch(lat, kFamSite);
     if (lat->prec == QCB_F64) {
-        launch_plaquette<double>(g, lat->d_partial);
-        launch_finalize_plaq<double>(f, lat->d_partial, d_out, 1);
+        launch_plaquette<double>(g, lat->d_partial, which, lat->gstep);
+        launch_finalize_plaq<double>(g, lat->d_partial, d_out, 1);
     } else {
-        launch_plaquette<float>(g, lat->d_partial);
-        launch_finalize_plaq<float>(f, lat->d_partial, d_out, 1);
+        launch_plaquette<float>(g, lat->d_partial, which, lat->gstep);
+        launch_finalize_plaq<float>(g, lat->d_partial, d_out, 1);
     }
     QCB_HIP(lat->ctx, hipGetLastError());
     return QCB_OK;
@@ -247,7 +281,7 @@ int enqueue_clover(qcb_lattice* lat, int which, double* d_out) {
 
 extern "C" int qcb_plaquette(qcb_lattice* lat, double* p) {
     if (!lat || !p) return QCB_EINVAL;
-    int rc = enqueue_plaquette(lat, lat->d_out);
+    int rc = enqueue_plaquette(lat, 0, lat->d_out);
     if (rc) return rc;
     QCB_HIP(lat->ctx, hipMemcpyAsync(p, lat->d_out, sizeof(double), hipMemcpyDeviceToHost, lat->ctx->stream));
     QCB_HIP(lat->ctx, hipStreamSynchronize(lat->ctx->stream));
@@ -271,13 +305,13 @@ extern "C" int qcb_staples(qcb_lattice* lat, double* out) {
     qcb_ctx* ctx = lat->ctx;
     size_t n = (size_t)lat->V * 72;
     double* d = nullptr;
-    QCB_HIP(ctx, hipMallocAsync((void**)&d, n * sizeof(double), ctx->stream));
+    DevTmp tmp(ctx);
+    if (int rc = tmp.alloc(n * sizeof(double), &d)) return rc;
     Launch g = lat_launch(lat, kFamLink);
     if (lat->prec == QCB_F64) launch_staples<double>(g, d);
     else launch_staples<float>(g, d);
     QCB_HIP(ctx, hipGetLastError());
     QCB_HIP(ctx, hipMemcpyAsync(out, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QCB_OK;
 }
@@ -290,9 +324,9 @@ int enqueue_hb_sweeps(qcb_lattice* lat, int n_hits, int n_sweeps, uint32_t sweep
         for (int mu = 0; mu < 4; ++mu)
             for (int p = 0; p < 2; ++p) {
                 if (lat->prec == QCB_F64)
-                    launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, lat->d_counter);
+                    launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0);
                 else
-                    launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, lat->d_counter);
+                    launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0);
             }
     QCB_HIP(lat->ctx, hipGetLastError());
     return QCB_OK;
@@ -302,13 +336,18 @@ extern "C" int qcb_hb_sweeps(qcb_lattice* lat, double beta, int n_hits, int n_sw
     if (!lat) return QCB_EINVAL;
     if (n_hits < 1 || n_hits > 4095 || n_sweeps < 0 || !(beta > 0))
         return qcb_set_error(lat->ctx, QCB_EINVAL, "need beta > 0, 1 <= n_hits < 4096, n_sweeps >= 0");
-    if (lat->beta != beta || lat->seed != seed) {
+    qcb_ctx* ctx = lat->ctx;
+    if (lat->beta != beta || lat->seed != seed || lat->kp_cap != ctx->kp_cap) {
         lat->beta = beta;
         lat->seed = seed;
+        lat->kp_cap = ctx->kp_cap;
         int rc = upload_desc(lat);
         if (rc) return rc;
     }
-    return enqueue_hb_sweeps(lat, n_hits, n_sweeps, sweep_idx);
+    QCB_HIP(ctx, hipMemsetAsync(lat->kp_exh, 0, sizeof(unsigned long long), ctx->stream));
+    int rc = enqueue_hb_sweeps(lat, n_hits, n_sweeps, sweep_idx);
+    if (rc) return rc;
+    return check_kp_counters(ctx, ctx->stream, lat->kp_exh, 1, lat->kp_cap);
 }
 
 extern "C" int qcb_hb_sweep_injected(qcb_lattice* lat, double beta, int n_hits, const double* u, int K, int64_t* n_exh) {
@@ -321,20 +360,20 @@ extern "C" int qcb_hb_sweep_injected(qcb_lattice* lat, double beta, int n_hits, 
     if (rc) return rc;
     size_t n = (size_t)8 * n_hits * 3 * (4 * K + 2) * lat->Vh;
     double* d = nullptr;
-    QCB_HIP(ctx, hipMallocAsync((void**)&d, n * sizeof(double), ctx->stream));
+    DevTmp tmp(ctx);
+    if (int rc2 = tmp.alloc(n * sizeof(double), &d)) return rc2;
     QCB_HIP(ctx, hipMemcpyAsync(d, u, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    QCB_HIP(ctx, hipMemsetAsync(lat->d_counter, 0, sizeof(unsigned long long), ctx->stream));
+    QCB_HIP(ctx, hipMemsetAsync(lat->kp_exh, 0, sizeof(unsigned long long), ctx->stream));
     Launch g = lat_launch(lat, kFamHalf);
     for (int mu = 0; mu < 4; ++mu)
         for (int p = 0; p < 2; ++p) {
             // the kernel indexes its half-sweep block with hs = 2*mu+p from the buffer start
-            if (lat->prec == QCB_F64) launch_hb_half<double>(g, mu, p, n_hits, 0, kHbInjected, d, K, lat->d_counter);
-            else launch_hb_half<float>(g, mu, p, n_hits, 0, kHbInjected, d, K, lat->d_counter);
+            if (lat->prec == QCB_F64) launch_hb_half<double>(g, mu, p, n_hits, 0, kHbInjected, d, K);
+            else launch_hb_half<float>(g, mu, p, n_hits, 0, kHbInjected, d, K);
         }
     QCB_HIP(ctx, hipGetLastError());
     unsigned long long h = 0;
-    QCB_HIP(ctx, hipMemcpyAsync(&h, lat->d_counter, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
+    QCB_HIP(ctx, hipMemcpyAsync(&h, lat->kp_exh, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (n_exh) *n_exh = (int64_t)h;
     return QCB_OK;
@@ -434,6 +473,16 @@ extern "C" int qcb_flow_qe(qcb_lattice* lat, double* Q, double* E) {
     return QCB_OK;
 }
 
+extern "C" int qcb_flow_plaquette(qcb_lattice* lat, double* p) {
+    if (!lat || !p) return QCB_EINVAL;
+    if (!lat->flow_active) return qcb_set_error(lat->ctx, QCB_ESTATE, "qcb_flow_copy must come first");
+    int rc = enqueue_plaquette(lat, 1, lat->d_out);
+    if (rc) return rc;
+    QCB_HIP(lat->ctx, hipMemcpyAsync(p, lat->d_out, sizeof(double), hipMemcpyDeviceToHost, lat->ctx->stream));
+    QCB_HIP(lat->ctx, hipStreamSynchronize(lat->ctx->stream));
+    return QCB_OK;
+}
+
 extern "C" int qcb_flow_get_links(qcb_lattice* lat, double* aos) {
     if (!lat || !aos) return QCB_EINVAL;
     if (!lat->flow_active) return qcb_set_error(lat->ctx, QCB_ESTATE, "qcb_flow_copy must come first");
@@ -448,7 +497,8 @@ extern "C" int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_
     if (rc) return rc;
     double* d = nullptr;
     size_t nmeas = (size_t)n_steps_gf + 1;
-    QCB_HIP(ctx, hipMallocAsync((void**)&d, nmeas * 2 * sizeof(double), ctx->stream));
+    DevTmp tmp(ctx);
+    if ((rc = tmp.alloc(nmeas * 2 * sizeof(double), &d))) return rc;
     std::vector<double> tv(nmeas);
     double t = 0.0;
     for (int s = 0; s < n_steps_gf; ++s) {
@@ -463,7 +513,6 @@ extern "C" int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_
     if (final_tQE && (rc = enqueue_clover(lat, 1, d + 2 * n_steps_gf))) return rc;
     std::vector<double> h(nmeas * 2);
     QCB_HIP(ctx, hipMemcpyAsync(h.data(), d, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(d, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     for (int s = 0; s < n_steps_gf; ++s) {
         tQE[3 * s] = tv[s];
@@ -490,15 +539,14 @@ extern "C" int qcb_math_probe(qcb_ctx* ctx, int prec, int op, int n, const doubl
     size_t win = op <= 2 ? 18 : op == 3 ? (size_t)(1 + 4 * aux + 2) : op == 4 ? (size_t)(36 + 3 * (4 * aux + 2)) : 1;
     size_t wout = op == 3 ? 5 : op == 5 ? 1 : 18;
     double *di = nullptr, *dout = nullptr;
-    QCB_HIP(ctx, hipMallocAsync((void**)&di, win * n * sizeof(double), ctx->stream));
-    QCB_HIP(ctx, hipMallocAsync((void**)&dout, wout * n * sizeof(double), ctx->stream));
+    DevTmp tin(ctx), tout(ctx);
+    if (int rc = tin.alloc(win * n * sizeof(double), &di)) return rc;
+    if (int rc = tout.alloc(wout * n * sizeof(double), &dout)) return rc;
     QCB_HIP(ctx, hipMemcpyAsync(di, in, win * n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
     if (prec == QCB_F64) launch_math_probe<double>(op, n, di, param, aux, dout, ctx->stream);
     else launch_math_probe<float>(op, n, di, param, aux, dout, ctx->stream);
     QCB_HIP(ctx, hipGetLastError());
     QCB_HIP(ctx, hipMemcpyAsync(out, dout, wout * n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(di, ctx->stream));
-    QCB_HIP(ctx, hipFreeAsync(dout, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return QCB_OK;
 }

@@ -29,7 +29,7 @@ struct qcb_batch {
     double* d_meas = nullptr;        // measurement buffer (grown on demand)
     size_t meas_cap = 0;
     uint32_t* d_sweep = nullptr;     // device sweep counter for graph replay
-    unsigned long long* d_counter = nullptr;
+    unsigned long long* d_counter = nullptr;   // n exhausted-draw counts, one per lattice
     uint32_t gstep = 0;
     // launch schedule (autotuned)
     int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};   // half-sweep order, hs = 2*mu + parity
@@ -96,13 +96,29 @@ int enqueue_batch_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_
         for (int k = 0; k < 8; ++k) {
             const int mu = b->order[k] >> 1, p = b->order[k] & 1;
             if (b->prec == QCB_F64)
-                launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, b->d_counter,
-                                       base, b->hb_grid);
+                launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, base,
+                                       b->hb_grid);
             else
-                launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, b->d_counter,
-                                      base, b->hb_grid);
+                launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, base,
+                                      b->hb_grid);
         }
     QCB_HIP(b->ctx, hipGetLastError());
+    return QCB_OK;
+}
+
+// Brings every lattice's descriptor to the context's attempt cap.
+int batch_sync_cap(qcb_batch* b) {
+    if (b->lats[0]->kp_cap == b->ctx->kp_cap) return QCB_OK;
+    for (qcb_lattice* l : b->lats) {
+        l->kp_cap = b->ctx->kp_cap;
+        int rc = upload_desc(l);
+        if (rc) return rc;
+    }
+    return batch_upload(b);
+}
+
+int zero_counters(qcb_batch* b) {
+    QCB_HIP(b->ctx, hipMemsetAsync(b->d_counter, 0, (size_t)b->n * sizeof(unsigned long long), b->ctx->stream));
     return QCB_OK;
 }
 
@@ -120,6 +136,12 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
     b->n = n;
     b->lats.assign(n, nullptr);
     int rc;
+    if (hipMalloc((void**)&b->d_counter, (size_t)n * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(b->d_counter, 0, (size_t)n * sizeof(unsigned long long)) != hipSuccess) {
+        batch_free(b);
+        delete b;
+        return qcb_set_error(ctx, QCB_ENOMEM, "cannot allocate the batch's counters");
+    }
     for (int i = 0; i < n; ++i) {
         rc = qcb_lattice_create(ctx, dims + 4 * i, prec, &b->lats[i]);
         if (rc) {
@@ -129,6 +151,7 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
         }
         b->lats[i]->beta = betas[i];
         b->lats[i]->seed = seeds[i];
+        b->lats[i]->kp_exh = b->d_counter + i;
         if ((rc = upload_desc(b->lats[i]))) {
             batch_free(b);
             delete b;
@@ -146,8 +169,6 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
     QCB_HIP(ctx, hipMalloc((void**)&b->d_partial, 2 * sizeof(double) * b->nb[kFamSite]));
     QCB_HIP(ctx, hipMalloc((void**)&b->d_sweep, sizeof(uint32_t)));
     QCB_HIP(ctx, hipMemset(b->d_sweep, 0, sizeof(uint32_t)));
-    QCB_HIP(ctx, hipMalloc((void**)&b->d_counter, sizeof(unsigned long long)));
-    QCB_HIP(ctx, hipMemset(b->d_counter, 0, sizeof(unsigned long long)));
     rc = batch_upload(b);
     if (rc) {
         batch_free(b);
@@ -161,6 +182,9 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
 
 extern "C" int qcb_batch_destroy(qcb_batch* b) {
     if (!b) return QCB_EINVAL;
+    for (qcb_lattice* l : b->lats)
+        if (l && l->n_users > 0)
+            return qcb_set_error(b->ctx, QCB_ESTATE, "event-chain handles still use a lattice of this batch");
     (void)hipStreamSynchronize(b->ctx->stream);
     b->ctx->n_live--;
     batch_free(b);
@@ -184,7 +208,10 @@ extern "C" int qcb_batch_hot(qcb_batch* b) {
 
 extern "C" int qcb_batch_hb_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx) {
     if (!b || n_hits < 1 || n_hits > 4095 || n_sweeps < 0) return QCB_EINVAL;
-    return enqueue_batch_sweeps(b, n_hits, n_sweeps, sweep_idx, nullptr);
+    int rc;
+    if ((rc = batch_sync_cap(b)) || (rc = zero_counters(b))) return rc;
+    if ((rc = enqueue_batch_sweeps(b, n_hits, n_sweeps, sweep_idx, nullptr))) return rc;
+    return check_kp_counters(b->ctx, b->ctx->stream, b->d_counter, b->n, b->ctx->kp_cap);
 }
 
 extern "C" int qcb_batch_reunitarize(qcb_batch* b) {
@@ -409,6 +436,8 @@ extern "C" int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_
         QCB_HIP(c, hipGraphDestroy(b->graph));
         b->graph = nullptr;
     }
+    int rc0 = batch_sync_cap(b);
+    if (rc0) return rc0;
     QCB_HIP(c, hipMemcpyAsync(b->d_sweep, &first_sweep, sizeof first_sweep, hipMemcpyHostToDevice, c->stream));
     QCB_HIP(c, hipStreamSynchronize(c->stream));
     QCB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -425,6 +454,8 @@ extern "C" int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_
 extern "C" int qcb_batch_graph_launch(qcb_batch* b, int n_replays) {
     if (!b || n_replays < 0) return QCB_EINVAL;
     if (!b->graph_exec) return qcb_set_error(b->ctx, QCB_ESTATE, "no captured graph");
+    int rc = zero_counters(b);
+    if (rc) return rc;
     for (int r = 0; r < n_replays; ++r) QCB_HIP(b->ctx, hipGraphLaunch(b->graph_exec, b->ctx->stream));
-    return QCB_OK;
+    return check_kp_counters(b->ctx, b->ctx->stream, b->d_counter, b->n, b->lats[0]->kp_cap);
 }

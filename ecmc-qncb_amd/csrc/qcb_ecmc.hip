@@ -494,7 +494,8 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
     const double beta_red = -(a.beta / 3.0);
     const double eps_a0 = sqrt(1 - a.epsilon_set * a.epsilon_set);   // a0 of every near-identity factor
     double par_sample = 0.0;
-    st.status = 0;
+    // status is sticky: a chain that stopped early in any launch keeps its
+    // non-zero status until qcb_ecmc_get_chains reports it (or set_chains)
     for (unsigned long long guard = 0;; ++guard) {
         if (guard >= a.max_events) {
             st.status = 1;
@@ -706,7 +707,8 @@ __global__ __launch_bounds__(kThreads) void k_ecmc_sample(double* __restrict__ E
 // ------------------------------------------------------------ C-ABI -----
 
 struct qcb_ecmc {
-    qcb_lattice* lat = nullptr;
+    qcb_ctx* ctx = nullptr;
+    qcb_lattice* lat = nullptr;   // counted in lat->n_users, so it cannot be destroyed first
     EcmcArgs args{};
     int n_chains = 0;
     EcmcChainDev* d_chains = nullptr;
@@ -743,6 +745,7 @@ extern "C" int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_e
     a.n_chains = n;
     a.bvol = (uint32_t)(a.b[0] * a.b[1] * a.b[2] * a.b[3]);
     qcb_ecmc* e = new qcb_ecmc();
+    e->ctx = ctx;
     e->lat = lat;
     e->args = a;
     e->n_chains = n;
@@ -757,16 +760,18 @@ extern "C" int qcb_ecmc_create(qcb_lattice* lat, const qcb_ecmc_params* p, qcb_e
                              link_bytes);
     }
     ctx->n_live++;
+    lat->n_users++;
     *out = e;
     return QCB_OK;
 }
 
 extern "C" int qcb_ecmc_destroy(qcb_ecmc* e) {
     if (!e) return QCB_EINVAL;
-    qcb_ctx* ctx = e->lat->ctx;
+    qcb_ctx* ctx = e->ctx;
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(e->d_chains);
     (void)hipFree(e->d_links);
+    e->lat->n_users--;
     ctx->n_live--;
     delete e;
     return QCB_OK;
@@ -781,7 +786,7 @@ extern "C" int qcb_ecmc_n_chains(qcb_ecmc* e, int* n) {
 extern "C" int qcb_ecmc_sample(qcb_ecmc* e, int n_samples) {
     if (!e || n_samples < 0) return QCB_EINVAL;
     qcb_lattice* lat = e->lat;
-    qcb_ctx* ctx = lat->ctx;
+    qcb_ctx* ctx = e->ctx;
     if (n_samples == 0) return QCB_OK;
     const int blocks = (e->n_chains * kTeam + kThreads - 1) / kThreads;
     const uint32_t n_links = (uint32_t)lat->V * 4u;
@@ -798,7 +803,7 @@ extern "C" int qcb_ecmc_sample(qcb_ecmc* e, int n_samples) {
 
 extern "C" int qcb_ecmc_get_chains(qcb_ecmc* e, double* out) {
     if (!e || !out) return QCB_EINVAL;
-    qcb_ctx* ctx = e->lat->ctx;
+    qcb_ctx* ctx = e->ctx;
     std::vector<EcmcChainDev> h(e->n_chains);
     QCB_HIP(ctx, hipMemcpyAsync(h.data(), e->d_chains, sizeof(EcmcChainDev) * h.size(), hipMemcpyDeviceToHost,
                                 ctx->stream));
@@ -814,16 +819,22 @@ extern "C" int qcb_ecmc_get_chains(qcb_ecmc* e, double* out) {
         o[28] = (double)s.site; o[29] = s.mu; o[30] = s.eps; o[31] = s.initialized;
         bad += s.status != 0;
     }
-    if (bad)
+    if (bad) {
+        // reported once: clear the statuses so the next call sees only new stops
+        for (EcmcChainDev& s : h) s.status = 0;
+        QCB_HIP(ctx, hipMemcpyAsync(e->d_chains, h.data(), sizeof(EcmcChainDev) * h.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
         return qcb_set_error(ctx, QCB_EBUDGET,
-                             "%d chains stopped before their sample event (max_events reached or a non-finite "
-                             "rejection angle)", bad);
+                             "%d chains stopped before their sample event since the last report (max_events "
+                             "reached or a non-finite rejection angle)", bad);
+    }
     return QCB_OK;
 }
 
 extern "C" int qcb_ecmc_set_chains(qcb_ecmc* e, const double* in) {
     if (!e || !in) return QCB_EINVAL;
-    qcb_ctx* ctx = e->lat->ctx;
+    qcb_ctx* ctx = e->ctx;
     const EcmcArgs& a = e->args;
     std::vector<EcmcChainDev> h(e->n_chains);
     for (int c = 0; c < e->n_chains; ++c) {

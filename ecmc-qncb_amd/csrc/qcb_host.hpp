@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/qcb.h"
 #include "qcb_internal.hpp"
@@ -13,6 +14,7 @@ struct qcb_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::string err;
     int n_live = 0;   // lattices and batches created on this context and not yet destroyed
+    uint32_t kp_cap = 1024;   // Kennedy-Pendleton attempts per draw for Philox heatbath sweeps
 };
 
 namespace qcb {
@@ -40,6 +42,9 @@ struct qcb_lattice {
     int tile[4] = {0, 1, 1, 1};  // work-order tile extents (tile[0] unused: whole x rows)
     uint32_t sweep_off = 0;      // per-lattice offset of the Philox sweep index
     int morton = 0;              // Z-order over tiles (needs power-of-two tile counts)
+    uint32_t kp_cap = 1024;      // attempt cap in the uploaded descriptor
+    unsigned long long* kp_exh = nullptr;   // where the heatbath counts exhausted draws (d_counter by default)
+    int n_users = 0;             // event-chain handles reading this lattice (it must outlive them)
 };
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
@@ -121,6 +126,8 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
     }
     d.tvol = (uint32_t)(l->L[0] / 2) * l->tile[1] * l->tile[2] * l->tile[3];
     d.morton = l->morton;
+    d.kp_cap = l->kp_cap;
+    d.kp_exh = l->kp_exh;
     d.tbits[0] = 0;
     for (int k = 1; k < 4; ++k) {
         int b = 0;
@@ -168,6 +175,8 @@ inline int lattice_alloc(qcb_lattice* l) {
     QCB_HIP(c, hipMalloc((void**)&l->d_out, 8 * sizeof(double)));
     QCB_HIP(c, hipMalloc((void**)&l->d_counter, sizeof(unsigned long long)));
     QCB_HIP(c, hipMemset(l->d_counter, 0, sizeof(unsigned long long)));
+    l->kp_exh = l->d_counter;
+    l->kp_cap = c->kp_cap;
     return upload_desc(l);
 }
 
@@ -180,6 +189,26 @@ inline int ensure_flow_buffers(qcb_lattice* l) {
                                  (unsigned long long)n);
     }
     return upload_desc(l);
+}
+
+// Reads n exhaustion counters after the work before it on `st` (synchronises)
+// and reports QCB_EBUDGET naming the first lattice whose count is non-zero.
+inline int check_kp_counters(qcb_ctx* c, hipStream_t st, const unsigned long long* d, int n, uint32_t cap) {
+    std::vector<unsigned long long> h((size_t)n, 0ull);
+    QCB_HIP(c, hipMemcpyAsync(h.data(), d, (size_t)n * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    QCB_HIP(c, hipStreamSynchronize(st));
+    unsigned long long tot = 0;
+    int first = -1;
+    for (int i = 0; i < n; ++i) {
+        tot += h[i];
+        if (h[i] && first < 0) first = i;
+    }
+    if (tot)
+        return qcb_set_error(c, QCB_EBUDGET,
+                             "%llu Kennedy-Pendleton draws ran out of their %u attempts (first in lattice %d); "
+                             "those SU(2) subgroup updates were skipped",
+                             tot, cap, first);
+    return QCB_OK;
 }
 
 inline void lattice_free(qcb_lattice* l) {

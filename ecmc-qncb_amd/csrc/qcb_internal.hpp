@@ -31,14 +31,16 @@ template <class T> void launch_be_to_soa(const Launch& g, const uint64_t* in, T*
 template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
 template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
 template <class T> void launch_flow_init(const Launch& g);   // F[0] <- U for every lattice
-template <class T> void launch_plaquette(const Launch& g, double* partial);
+// which = 0: the links; 1: each lattice's current flow field at flow step gstep
+template <class T> void launch_plaquette(const Launch& g, double* partial, int which = 0, uint32_t gstep = 0);
 template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
 template <class T> void launch_staples(const Launch& g, double* out_aos);
 // sweep_base: optional device counter added to sweep_idx (graph replay);
-// grid: 0 = one block per tile, else a persistent grid (a multiple of 8)
+// grid: 0 = one block per tile, else a persistent grid (a multiple of 8).
+// Exhausted Kennedy-Pendleton draws are added to each lattice's kp_exh.
 template <class T> void launch_hb_half(const Launch& g, int mu, int parity, int n_hits, uint32_t sweep_idx,
-                                       int mode, const double* inj, int K, unsigned long long* exhausted,
-                                       const uint32_t* sweep_base = nullptr, uint32_t grid = 0);
+                                       int mode, const double* inj, int K, const uint32_t* sweep_base = nullptr,
+                                       uint32_t grid = 0);
 void launch_bump(uint32_t* counter, uint32_t n, hipStream_t st);
 template <class T> void launch_reunit(const Launch& g);
 template <class T> void launch_translate(const Launch& g, const int s[4], T* dst_field);

@@ -251,10 +251,12 @@ __global__ __launch_bounds__(kBlock) void k_reunit(const LatDev<T>* __restrict__
 // ------------------------------------------------------------ plaquette ----
 
 // Per-site sum over mu<nu of Re Tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag)/3
-// (reference src/observables/observables_mpi.cpp:8-30), FP64 accumulation.
+// (reference src/observables/observables_mpi.cpp:8-30), FP64 accumulation,
+// of the links (which = 0) or of the lattice's current flow field (which = 1).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                      int n_lat, uint32_t nblocks, double* __restrict__ partial) {
+                                                      int n_lat, uint32_t nblocks, int which, uint32_t gstep,
+                                                      double* __restrict__ partial) {
     __shared__ double lds[kBlock / 64];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
     if (qs < L.Vh) {
         Coords c = tiled_coords(L, qs, p);
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-        const T* f = L.U;
+        const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
 #pragma unroll
         for (int mu = 0; mu < 3; ++mu) {
             M3<T> umu = ld_link(f, L.S, mu, p, i);
@@ -526,20 +528,19 @@ __device__ __forceinline__ void kp_sphere(T a0, T uphi, T ucos, T x[4]) {
     x[3] = rs * ct;
 }
 
-constexpr uint32_t kPhiloxMaxAttempts = 1024;
-
 // Cabibbo-Marinari hit on U with staple A, subgroups (0,1),(1,2),(0,2)
 // (reference src/heatbath/heatbath_mpi.cpp:11-34).  Only the four entries of
 // W = U A in the subgroup block are formed, and only rows i, j of U change.
 // Sphere uniforms (Philox mode): subgroup m = 3h + g uses words 2(m&1) and
 // 2(m&1)+1 of the call 0x80000000 at the counter of subgroup m - (m&1), so
 // two subgroups share one call; `stash` carries the odd member's two words
-// from one subgroup (or hit) to the next.  Returns the number of exhausted
-// KP budgets.
+// from one subgroup (or hit) to the next.  A draw gets cap attempts in
+// Philox mode and K in injected mode; one that runs out leaves the link's
+// subgroup unchanged.  Returns the number of such exhausted draws.
 template <class T, int MODE>
 __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, int h, uint32_t cb, uint32_t sweep,
                                       uint32_t key0, uint32_t key1, const double* __restrict__ inj, int n_hits, int K,
-                                      uint32_t Vh, uint32_t stash[2]) {
+                                      uint32_t Vh, uint32_t stash[2], uint32_t cap) {
     int exhausted = 0;
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
@@ -601,7 +602,7 @@ __device__ __forceinline__ int cm_hit(M3<T>& U, const M3<T>& A, T beta, int hs, 
                 }
             }
         } else {
-            for (uint32_t a = 0; a < kPhiloxMaxAttempts; ++a) {
+            for (uint32_t a = 0; a < cap; ++a) {
                 U4 o = philox4x32_10(U4{cb, sweep, cz, a}, key0, key1);
                 if (kp_try<T>(nhia, KpMath::uni<T>(o.x), KpMath::uni<T>(o.y), KpMath::uni<T>(o.z),
                               KpMath::uni<T>(o.w), a0)) {
@@ -651,8 +652,7 @@ template <class T, int MU, int MODE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::hb))) void k_hb_half(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                     int n_lat, uint32_t nblocks, int p, int n_hits, uint32_t sweep,
                                                     const uint32_t* __restrict__ sweep_base,
-                                                    const double* __restrict__ inj, int K,
-                                                    unsigned long long* __restrict__ exhausted) {
+                                                    const double* __restrict__ inj, int K) {
     // sweep_base (device counter, advanced by k_bump) lets a captured graph of
     // sweeps be replayed with fresh Philox counters
     if (sweep_base) sweep += *sweep_base;
@@ -673,10 +673,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         int ex = 0;
         uint32_t stash[2] = {0u, 0u};
         for (int h = 0; h < n_hits; ++h)
-            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh, stash);
+            ex += cm_hit<T, MODE>(U, A, beta, 2 * MU + p, h, i, sw, L.key0, L.key1, inj, n_hits, K, L.Vh, stash,
+                                  L.kp_cap);
         su3_store_fix(U);
         st_link(L.U, L.S, MU, p, i, U);
-        if (ex) atomicAdd(exhausted, (unsigned long long)ex);
+        if (ex && L.kp_exh) atomicAdd(L.kp_exh, (unsigned long long)ex);
     }
 }
 
@@ -880,7 +881,7 @@ __global__ void k_math_probe(int op, int n, const double* __restrict__ in, doubl
         for (int k = 0; k < 18; ++k) { U.v[k] = (T)src[k]; A.v[k] = (T)src[18 + k]; }
         // uniforms laid out [g][slot] with stride 1: reuse the injected path with Vh = 1
         uint32_t stash[2] = {0u, 0u};
-        cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1, stash);
+        cm_hit<T, kHbInjected>(U, A, (T)param, 0, 0, 0, 0, 0, 0, src + 36, 1, K, 1, stash, (uint32_t)K);
         for (int k = 0; k < 18; ++k) out[(size_t)t * 18 + k] = (double)U.v[k];
     } else if (op == 5) {
         out[t] = (double)KpMath::uni<T>((uint32_t)in[t]);
@@ -928,8 +929,9 @@ void launch_flow_init(const Launch& g) {
     hipLaunchKernelGGL(k_flow_init<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
 }
 template <class T>
-void launch_plaquette(const Launch& g, double* partial) {
-    hipLaunchKernelGGL(k_plaquette<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), partial);
+void launch_plaquette(const Launch& g, double* partial, int which, uint32_t gstep) {
+    hipLaunchKernelGGL(k_plaquette<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), which, gstep,
+                       partial);
 }
 template <class T>
 void launch_clover(const Launch& g, int which, uint32_t gstep, double* pq, double* pe) {
@@ -951,24 +953,24 @@ void launch_translate(const Launch& g, const int s[4], T* dst) {
 
 template <class T, int MU>
 static void hb_mu(const Launch& g, int p, int n_hits, uint32_t sweep, const uint32_t* base, int mode, const double* inj,
-                  int K, unsigned long long* ex, uint32_t grid) {
+                  int K, uint32_t grid) {
     if (mode == kHbInjected)
         hipLaunchKernelGGL((k_hb_half<T, MU, kHbInjected>), dim3(grid), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
-                           n_hits, sweep, base, inj, K, ex);
+                           n_hits, sweep, base, inj, K);
     else
         hipLaunchKernelGGL((k_hb_half<T, MU, kHbPhilox>), dim3(grid), dim3(kBlock), 0, g.stream, QCB_ARGS(g), p,
-                           n_hits, sweep, base, inj, K, ex);
+                           n_hits, sweep, base, inj, K);
 }
 
 template <class T>
 void launch_hb_half(const Launch& g, int mu, int p, int n_hits, uint32_t sweep, int mode, const double* inj, int K,
-                    unsigned long long* ex, const uint32_t* sweep_base, uint32_t grid) {
+                    const uint32_t* sweep_base, uint32_t grid) {
     if (grid == 0 || grid > g.nblocks) grid = g.nblocks;
     switch (mu) {
-        case 0: hb_mu<T, 0>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
-        case 1: hb_mu<T, 1>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
-        case 2: hb_mu<T, 2>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
-        default: hb_mu<T, 3>(g, p, n_hits, sweep, sweep_base, mode, inj, K, ex, grid); break;
+        case 0: hb_mu<T, 0>(g, p, n_hits, sweep, sweep_base, mode, inj, K, grid); break;
+        case 1: hb_mu<T, 1>(g, p, n_hits, sweep, sweep_base, mode, inj, K, grid); break;
+        case 2: hb_mu<T, 2>(g, p, n_hits, sweep, sweep_base, mode, inj, K, grid); break;
+        default: hb_mu<T, 3>(g, p, n_hits, sweep, sweep_base, mode, inj, K, grid); break;
     }
 }
 
@@ -1011,13 +1013,13 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_soa_to_be<T>(const Launch&, const T*, uint64_t*, uint64_t, uint32_t);                 \
     template void launch_be_to_soa<T>(const Launch&, const uint64_t*, T*, uint64_t, uint32_t);                 \
     template void launch_flow_init<T>(const Launch&);                                                         \
-    template void launch_plaquette<T>(const Launch&, double*);                                                \
+    template void launch_plaquette<T>(const Launch&, double*, int, uint32_t);                                                \
     template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
     template void launch_staples<T>(const Launch&, double*);                                                       \
     template void launch_reunit<T>(const Launch&);                                                            \
     template void launch_translate<T>(const Launch&, const int*, T*);                                         \
     template void launch_hb_half<T>(const Launch&, int, int, int, uint32_t, int, const double*, int,           \
-                                    unsigned long long*, const uint32_t*, uint32_t);                          \
+                                    const uint32_t*, uint32_t);                                               \
     template void launch_flow_stage<T>(const Launch&, int, uint32_t, double);                                 \
     template void launch_math_probe<T>(int, int, const double*, double, int, double*, hipStream_t);           \
     template void launch_finalize_plaq<T>(const Launch&, const double*, double*, int);                        \

@@ -49,6 +49,8 @@ struct LatDev {
     uint32_t tvol;        // cb sites per tile = (Lx/2) * T1 * T2 * T3
     int morton;           // tiles in Z-order over (ty, tz, tt) (NT powers of two) instead of y-fastest
     int tbits[4];         // log2 NT[d] when morton
+    uint32_t kp_cap;      // Kennedy-Pendleton attempts per subgroup draw in Philox mode
+    unsigned long long* kp_exh;  // device count of draws that ran out of attempts (skipped updates)
 };
 
 // Threads per block of each kernel family and the work item of a thread:

@@ -42,7 +42,8 @@ struct Req {
     int sweeps_done = 0;
     int flow_step = 0;
     double t_flow = 0.0;
-    size_t off = 0, bytes = 0;  // arena slice
+    size_t off = 0, bytes = 0;  // arena slice: links, flow fields, then the exhausted-draw counter
+    unsigned long long kp_exhausted = 0;
     std::vector<double> tQE;
 };
 
@@ -114,7 +115,6 @@ struct qcb_service {
     double* d_res = nullptr;
     size_t res_cap = 0;
     hipStream_t s_hb = nullptr, s_fl = nullptr;
-    unsigned long long* d_counter = nullptr;
     uint32_t gstep = 0;
     std::chrono::steady_clock::time_point t0;
     uint64_t n_iter = 0;
@@ -173,7 +173,8 @@ void devset_free(DevSet& d) {
     for (int f = 0; f < 3; ++f) (void)hipFree(d.d_blk[f]);
 }
 
-uint64_t req_bytes(const qcb_lattice& l) { return field_bytes(&l) + flow_bytes(&l); }
+// links, the flow fields and a 256-byte tail for the request's counter
+uint64_t req_bytes(const qcb_lattice& l) { return field_bytes(&l) + flow_bytes(&l) + 256; }
 
 }  // namespace
 
@@ -199,8 +200,6 @@ extern "C" int qcb_service_create(qcb_ctx* ctx, int prec, uint64_t arena_bytes, 
     s->arena.init(arena_bytes);
     QCB_HIP(ctx, hipStreamCreateWithFlags(&s->s_hb, hipStreamNonBlocking));
     QCB_HIP(ctx, hipStreamCreateWithFlags(&s->s_fl, hipStreamNonBlocking));
-    QCB_HIP(ctx, hipMalloc((void**)&s->d_counter, sizeof(unsigned long long)));
-    QCB_HIP(ctx, hipMemset(s->d_counter, 0, sizeof(unsigned long long)));
     ctx->n_live++;
     *out = s;
     return QCB_OK;
@@ -221,7 +220,6 @@ extern "C" int qcb_service_destroy(qcb_service* s) {
     devset_free(s->set_fs);
     (void)hipFree(s->d_partial);
     (void)hipFree(s->d_res);
-    (void)hipFree(s->d_counter);
     (void)hipFree(s->arena_base);
     (void)hipStreamDestroy(s->s_hb);
     (void)hipStreamDestroy(s->s_fl);
@@ -283,6 +281,9 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
             const uint64_t fb = field_bytes(&r->lat);
             r->lat.U = base;
             for (int k = 0; k < 3; ++k) r->lat.F[k] = base + (k + 1) * fb;   // F[2] holds force_bytes
+            r->lat.kp_exh = (unsigned long long*)(base + fb + flow_bytes(&r->lat));
+            r->lat.kp_cap = c->kp_cap;
+            QCB_HIP(c, hipMemsetAsync(r->lat.kp_exh, 0, sizeof(unsigned long long), s->s_hb));
             r->state = 1;
             r->fresh = true;
             r->t_admit = now_s(s);
@@ -339,8 +340,8 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
     if (!hb.empty()) {
         Launch g = set_launch(s->set_hb, kFamHalf, s->s_hb);
         for (int hs = 0; hs < 8; ++hs) {
-            if (f64) launch_hb_half<double>(g, hs >> 1, hs & 1, s->n_hits, 0, kHbPhilox, nullptr, 0, s->d_counter);
-            else launch_hb_half<float>(g, hs >> 1, hs & 1, s->n_hits, 0, kHbPhilox, nullptr, 0, s->d_counter);
+            if (f64) launch_hb_half<double>(g, hs >> 1, hs & 1, s->n_hits, 0, kHbPhilox, nullptr, 0);
+            else launch_hb_half<float>(g, hs >> 1, hs & 1, s->n_hits, 0, kHbPhilox, nullptr, 0);
         }
         for (Req* r : hb) r->sweeps_done++;
     }
@@ -401,6 +402,7 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
     for (auto it = s->active.begin(); it != s->active.end();) {
         Req* r = *it;
         if (r->state == 3) {
+            QCB_HIP(c, hipMemcpy(&r->kp_exhausted, r->lat.kp_exh, sizeof(unsigned long long), hipMemcpyDeviceToHost));
             s->arena.release(r->off, r->bytes);
             s->done.push_back(r);
             it = s->active.erase(it);
@@ -431,7 +433,13 @@ extern "C" int qcb_service_poll(qcb_service* s, uint64_t* id, double* tQE, int c
         times3[1] = r->t_admit;
         times3[2] = r->t_done;
     }
+    const unsigned long long exh = r->kp_exhausted;
     delete r;
+    if (exh)
+        return qcb_set_error(s->ctx, QCB_EBUDGET,
+                             "request %llu: %llu Kennedy-Pendleton draws ran out of attempts during thermalisation "
+                             "(results returned)",
+                             (unsigned long long)*id, exh);
     return QCB_OK;
 }
 

@@ -58,6 +58,7 @@ ECMC_FIELDS = [f"R{k}" for k in range(18)] + [
 SIGNATURES = [
     ("qcb_ctx_create", C.c_int, [C.c_int, C.POINTER(_VP)]),
     ("qcb_ctx_destroy", C.c_int, [_VP]),
+    ("qcb_ctx_set_kp_budget", C.c_int, [_VP, C.c_uint32]),
     ("qcb_last_error", C.c_char_p, [_VP]),
     ("qcb_ctx_synchronize", C.c_int, [_VP]),
     ("qcb_timing_begin", C.c_int, [_VP]),
@@ -81,6 +82,7 @@ SIGNATURES = [
     ("qcb_flow_copy", C.c_int, [_VP]),
     ("qcb_flow_rk3", C.c_int, [_VP, C.c_double, C.c_int]),
     ("qcb_flow_qe", C.c_int, [_VP, _D, _D]),
+    ("qcb_flow_plaquette", C.c_int, [_VP, _D]),
     ("qcb_flow_get_links", C.c_int, [_VP, _D]),
     ("qcb_topo_charge_flowed", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
     ("qcb_batch_create", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.c_int, _D, C.POINTER(C.c_uint64),
@@ -158,7 +160,10 @@ class Context:
 
     def close(self):
         if self.h:
-            for child in list(self._children):
+            # event chains read their lattice (the library refuses to destroy a
+            # lattice that chains still use), so they go first
+            children = list(self._children)
+            for child in [c for c in children if isinstance(c, Ecmc)] + [c for c in children if not isinstance(c, Ecmc)]:
                 child.close()
             self.check(lib().qcb_ctx_destroy(self.h))
             self.h = _VP()
@@ -171,6 +176,10 @@ class Context:
 
     def sync(self):
         self.check(lib().qcb_ctx_synchronize(self.h))
+
+    def set_kp_budget(self, max_attempts: int):
+        """Kennedy-Pendleton attempts per draw for Philox heatbath sweeps (default 1024)."""
+        self.check(lib().qcb_ctx_set_kp_budget(self.h, max_attempts))
 
     def timing_begin(self):
         self.check(lib().qcb_timing_begin(self.h))
@@ -198,12 +207,15 @@ class Lattice:
         self.prec = prec
         self.V = int(np.prod(self.dims))
         self.h = _VP()
+        self._users = weakref.WeakSet()   # event chains on this lattice
         ctx.check(lib().qcb_lattice_create(ctx.h, (C.c_int * 4)(*self.dims), prec, C.byref(self.h)))
         ctx._children.add(self)
 
     def close(self):
+        for u in list(self._users):
+            u.close()
         if self.h and self.ctx.h:
-            lib().qcb_lattice_destroy(self.h)
+            self.ctx.check(lib().qcb_lattice_destroy(self.h))
         self.h = _VP()
 
     def __del__(self):
@@ -289,6 +301,11 @@ class Lattice:
         q, e = C.c_double(), C.c_double()
         self._c(lib().qcb_flow_qe(self.h, C.byref(q), C.byref(e)))
         return q.value, e.value
+
+    def flow_plaquette(self) -> float:
+        p = C.c_double()
+        self._c(lib().qcb_flow_plaquette(self.h, C.byref(p)))
+        return p.value
 
     def flow_links(self) -> np.ndarray:
         out = np.zeros(self.V * 72)
@@ -411,6 +428,8 @@ class Ecmc:
         self.ctx = lat.ctx
         self.h = _VP()
         self.ctx.check(lib().qcb_ecmc_create(lat.h, C.byref(params), C.byref(self.h)))
+        if hasattr(lat, "_users"):
+            lat._users.add(self)
         n = C.c_int()
         self.ctx.check(lib().qcb_ecmc_n_chains(self.h, C.byref(n)))
         self.n_chains = n.value
@@ -479,7 +498,10 @@ class Service:
         return p.value
 
     def poll(self, cap=4096):
-        """Finished requests as a list of (id, tQE[n,3], (arrival, admit, done))."""
+        """Finished requests as a list of (id, tQE[n,3], (arrival, admit, done)).
+        A request whose thermalisation skipped subgroup updates (Kennedy-Pendleton
+        budget) raises QcbError(EBUDGET) carrying it as .result and the requests
+        popped before it as .done."""
         out = []
         while True:
             rid, n = C.c_uint64(), C.c_int()
@@ -488,6 +510,11 @@ class Service:
             rc = lib().qcb_service_poll(self.h, C.byref(rid), _dp(buf), cap, C.byref(n), _dp(times))
             if rc == ESTATE:
                 return out
+            if rc == EBUDGET:
+                err = QcbError(rc, lib().qcb_last_error(self.ctx.h).decode())
+                err.result = (rid.value, buf[: 3 * n.value].reshape(n.value, 3), tuple(times))
+                err.done = out
+                raise err
             self.ctx.check(rc)
             out.append((rid.value, buf[: 3 * n.value].reshape(n.value, 3), tuple(times)))
 

@@ -56,6 +56,13 @@ int qcb_ctx_destroy(qcb_ctx* ctx);
 const char* qcb_last_error(qcb_ctx* ctx);
 int qcb_ctx_synchronize(qcb_ctx* ctx);
 /* qcb_last_error(NULL) reports the last failure of a context-free call. */
+/* Attempts a Kennedy-Pendleton draw gets in Philox-mode heatbath sweeps on
+ * this context from the next sweep call on (default 1024, 1..2^20).  The
+ * reference loops without bound (src/heatbath/heatbath.cpp:14-26); a draw
+ * that runs out leaves its link's SU(2) subgroup unchanged, and the sweep call
+ * that made it returns QCB_EBUDGET (its other updates stand).  At beta = 6 a
+ * draw needs more than 10 attempts about once in 10^9. */
+int qcb_ctx_set_kp_budget(qcb_ctx* ctx, uint32_t max_attempts);
 /* Device-side time of the last kernel-timing window (see qcb_timing_*), ms. */
 int qcb_timing_begin(qcb_ctx* ctx);
 int qcb_timing_end(qcb_ctx* ctx, double* ms);
@@ -104,7 +111,8 @@ int qcb_staples(qcb_lattice* lat, double* out_aos);
  * Cabibbo-Marinari hits (src/heatbath/heatbath_mpi.cpp:11-34) with
  * Kennedy-Pendleton SU(2) draws (src/heatbath/heatbath.cpp:8-44).  Uniforms
  * come from Philox keyed by seed with counter sweep_idx, sweep_idx+1, ...
- * (a Philox stream replaces the std::mt19937_64& argument). */
+ * (a Philox stream replaces the std::mt19937_64& argument).  Synchronises;
+ * QCB_EBUDGET if any draw ran out of attempts (qcb_ctx_set_kp_budget). */
 int qcb_hb_sweeps(qcb_lattice* lat, double beta, int n_hits, int n_sweeps, uint64_t seed,
                   uint32_t sweep_idx);
 /* One sweep fed with host-supplied uniforms: u[((((2*mu+p)*n_hits + h)*3 + g)*(4K+2)
@@ -132,6 +140,8 @@ int qcb_flow_steps(qcb_lattice* lat, double eps, int n);
 int qcb_flow_copy(qcb_lattice* lat);
 int qcb_flow_rk3(qcb_lattice* lat, double eps, int n);
 int qcb_flow_qe(qcb_lattice* lat, double* Q, double* E_mean);
+/* mean_plaquette_global (src/observables/observables_mpi.cpp:34-43) of field_c. */
+int qcb_flow_plaquette(qcb_lattice* lat, double* mean_plaquette);
 int qcb_flow_get_links(qcb_lattice* lat, double* aos);
 /* mpi::observables::topo_charge_flowed (src/observables/observables_mpi.cpp:153-178):
  * on a copy of the links, for s < n_steps_gf record (t, Q, E_mean) into
@@ -154,6 +164,7 @@ int qcb_batch_destroy(qcb_batch* b);
 /* Borrowed handle of lattice i (set/get links, observables); owned by the batch. */
 int qcb_batch_lattice(qcb_batch* b, int i, qcb_lattice** out);
 int qcb_batch_hot(qcb_batch* b);
+/* Synchronises; QCB_EBUDGET (naming the first lattice) if any draw ran out of attempts. */
 int qcb_batch_hb_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx);
 int qcb_batch_reunitarize(qcb_batch* b);
 int qcb_batch_plaquette(qcb_batch* b, double* out /* n */);
@@ -173,6 +184,7 @@ int qcb_batch_get_schedule(qcb_batch* b, int order[8], uint32_t* hb_grid);
 int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len);
 /* hipGraph of sweeps_per_graph sweeps; replays continue the Philox sweep index. */
 int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_graph, uint32_t first_sweep);
+/* Synchronises; QCB_EBUDGET as qcb_batch_hb_sweeps. */
 int qcb_batch_graph_launch(qcb_batch* b, int n_replays);
 
 /* ---- flow-measurement service (continuous batching) ---------------------- */
@@ -197,7 +209,9 @@ int qcb_service_submit(qcb_service* s, uint64_t id, const int dims[4], double be
 /* One iteration: admit, advance every resident request, retire finished ones. */
 int qcb_service_step(qcb_service* s, int* pending);
 /* Pops one finished request (QCB_ESTATE if none): tQE holds n_meas triples;
- * times3 = (arrival, admit, done) on the service clock. */
+ * times3 = (arrival, admit, done) on the service clock.  QCB_EBUDGET (with
+ * every output filled in) if a heatbath draw of its thermalisation ran out of
+ * attempts. */
 int qcb_service_poll(qcb_service* s, uint64_t* id, double* tQE, int cap, int* n_meas, double* times3);
 int qcb_service_status(qcb_service* s, int* queued, int* resident, int* finished, uint64_t* free_bytes,
                        uint64_t* iterations);

@@ -66,6 +66,7 @@ typedef struct {
     double* U;         /* V*4*18 doubles */
 } qo_lat;
 
+void qo_set_threads(int n);                                             /* OpenMP threads of the per-site loops */
 int  qo_lat_init(qo_lat* l, const int L[4]);
 void qo_lat_free(qo_lat* l);
 void qo_lat_cold(qo_lat* l);                                            /* GaugeField.cpp:23-33 */

@@ -7,6 +7,19 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the oracle's data-parallel loops (flow, plaquette, clover);
+ * results do not depend on it: every global sum runs sequentially. */
+void qo_set_threads(int n) {
+#ifdef _OPENMP
+    omp_set_num_threads(n > 0 ? n : 1);
+#else
+    (void)n;
+#endif
+}
 
 void qo_m3_dag(const double* a, double* out);
 void qo_kp_mt(double beta, double k, qo_mt64* g, double out[4]);
@@ -284,8 +297,12 @@ void qo_staple(const qo_lat* l, size_t x, int mu, double* st) {
 /* <P> = sum_{x, mu<nu} Re Tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag)/3 / (6V)
  * (src/observables/observables_mpi.cpp:8-43), sequential sum in t,z,y,x,mu,nu order. */
 double qo_plaquette(const qo_lat* l) {
-    double sum = 0.0, a[18], b[18], c[18];
-    for (size_t s = 0; s < l->V; ++s)
+    /* the terms are formed in parallel and summed in the sequential order */
+    double* term = (double*)malloc(l->V * 6 * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (size_t s = 0; s < l->V; ++s) {
+        double a[18], b[18], c[18];
+        int k = 0;
         for (int mu = 0; mu < 4; ++mu)
             for (int nu = mu + 1; nu < 4; ++nu) {
                 qo_m3_mul(LNK(l, s, mu), LNK(l, nb(l, s, mu, +1), nu), a);
@@ -293,8 +310,12 @@ double qo_plaquette(const qo_lat* l) {
                 qo_m3_mul(a, b, c);
                 qo_m3_dag(LNK(l, s, nu), b);
                 qo_m3_mul(c, b, a);
-                sum += (a[0] + a[8] + a[16]) / 3.0;
+                term[s * 6 + k++] = (a[0] + a[8] + a[16]) / 3.0;
             }
+    }
+    double sum = 0.0;
+    for (size_t k = 0; k < l->V * 6; ++k) sum += term[k];
+    free(term);
     return sum / (6.0 * (double)l->V);
 }
 
@@ -350,18 +371,22 @@ static void qe_site(const qo_lat* l, size_t s, double* q, double* e) {
 }
 
 void qo_clover_qe_site(const qo_lat* l, double* q_site, double* e_site) {
+#pragma omp parallel for schedule(static)
     for (size_t s = 0; s < l->V; ++s) qe_site(l, s, &q_site[s], &e_site[s]);
 }
 
-/* (Q, E/V) with Q = sum_x q_x, E = sum_x e_x (src/observables/observables_mpi.cpp:88-150). */
+/* (Q, E/V) with Q = sum_x q_x, E = sum_x e_x (src/observables/observables_mpi.cpp:88-150);
+ * site densities in parallel, sums in the sequential site order. */
 void qo_clover_qe(const qo_lat* l, double* Q, double* E_mean) {
+    double* qs = (double*)malloc(l->V * 2 * sizeof(double));
+    double* es = qs + l->V;
+    qo_clover_qe_site(l, qs, es);
     double q = 0.0, e = 0.0;
     for (size_t s = 0; s < l->V; ++s) {
-        double qs, es;
-        qe_site(l, s, &qs, &es);
-        q += qs;
-        e += es;
+        q += qs[s];
+        e += es[s];
     }
+    free(qs);
     *Q = q;
     *E_mean = e / (double)l->V;
 }
@@ -404,9 +429,10 @@ void qo_gauge_transform(qo_lat* l, const double* G) {
 
 /* Z = P_TA(-U Sigma) for every link (src/flow/gradient_flow.cpp:15-34). */
 static void compute_force(const qo_lat* l, double* F) {
-    double st[18], t[18];
+#pragma omp parallel for schedule(static)
     for (size_t s = 0; s < l->V; ++s)
         for (int mu = 0; mu < 4; ++mu) {
+            double st[18], t[18];
             qo_staple(l, s, mu, st);
             qo_m3_mul(LNK(l, s, mu), st, t);
             for (int k = 0; k < 18; ++k) t[k] = -t[k];
@@ -421,22 +447,30 @@ void qo_flow_rk3_step(qo_lat* l, double eps) {
     size_t n = l->V * 72;
     double* F0 = (double*)malloc(n * sizeof(double));
     double* F1 = (double*)malloc(n * sizeof(double));
-    double Z[18], E[18];
+    /* every stage reads the forces of the whole field before any link moves,
+     * so links are updated in parallel */
     compute_force(l, F0);
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < l->V * 4; ++k) {
+        double E[18];
         qo_exp_luscher(F0 + k * 18, 0.25 * eps, E);
         qo_m3_mul(E, l->U + k * 18, l->U + k * 18);
     }
     compute_force(l, F1);
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < l->V * 4; ++k) {
+        double Z[18], E[18];
         for (int c = 0; c < 18; ++c)
             Z[c] = (8.0 / 9.0) * eps * F1[k * 18 + c] - (17.0 / 36.0) * eps * F0[k * 18 + c];
         qo_exp_luscher(Z, 1.0, E);
         qo_m3_mul(E, l->U + k * 18, l->U + k * 18);
     }
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < n; ++k) F0[k] = 8.0 / 9.0 * F1[k] - (17.0 / 36.0) * F0[k];
     compute_force(l, F1);
+#pragma omp parallel for schedule(static)
     for (size_t k = 0; k < l->V * 4; ++k) {
+        double Z[18], E[18];
         for (int c = 0; c < 18; ++c) Z[c] = 0.75 * eps * F1[k * 18 + c] - eps * F0[k * 18 + c];
         qo_exp_luscher(Z, 1.0, E);
         qo_m3_mul(E, l->U + k * 18, l->U + k * 18);

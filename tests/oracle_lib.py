@@ -75,6 +75,7 @@ def lib():
         P = C.POINTER
         sig = {
             "qo_m3_mul": (None, [_D, _D, _D]),
+            "qo_set_threads": (None, [C.c_int]),
             "qo_proj_lie": (None, [_D]),
             "qo_exp_luscher": (None, [_D, C.c_double, _D]),
             "qo_su2q_to_su3": (None, [_D, C.c_int, C.c_int, _D]),

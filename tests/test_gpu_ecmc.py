@@ -187,3 +187,65 @@ def test_equilibrium_plaquette_matches_heatbath(ctx):
     sigma = np.hypot(err, e_hb)
     print(f"ECMC <P> = {m:.6f} +- {err:.6f}, heatbath {m_hb:.6f} +- {e_hb:.6f}, {abs(m - m_hb) / sigma:.2f} sigma")
     assert abs(m - m_hb) < 3 * sigma
+
+
+def _stopped(err):
+    return int(str(err).split("qcb error -4: ")[1].split()[0])
+
+
+def test_early_stop_status_is_sticky(ctx):
+    """A chain that stops early (max_events) in any sample launch is reported
+    by the next qcb_ecmc_get_chains, not only when it stops in the last one;
+    a report clears it.  Two handles on identical lattices follow the same
+    trajectories: reading after each of two samples gives the stop sets S1 and
+    S2, reading once after both must give |S1 u S2|."""
+    dims, kw = (16, 16, 8, 8), dict(beta=6.0, theta_sample=12.3, theta_refresh_site=5.1, theta_refresh_R=3.7,
+                                    poisson=True, block=(4, 4, 4, 4))
+    g0 = qcb.Lattice(ctx, dims)
+    g0.hot(3)
+    e0 = qcb.Ecmc(g0, qcb.ecmc_params(seed=77, **kw))
+    e0.sample(1)
+    M = int(np.median(e0.chains()[:, F["events"]]))
+    e0.close()
+    g0.close()
+    gA, gB = qcb.Lattice(ctx, dims), qcb.Lattice(ctx, dims)
+    gA.hot(3)
+    gB.hot(3)
+    eA = qcb.Ecmc(gA, qcb.ecmc_params(seed=77, max_events=M, **kw))
+    eB = qcb.Ecmc(gB, qcb.ecmc_params(seed=77, max_events=M, **kw))
+    eA.sample(2)
+    with pytest.raises(qcb.QcbError) as e:
+        eA.chains()
+    nA = _stopped(e.value)
+    eA.chains()                       # reported once, then cleared
+    b = []
+    for _ in range(2):
+        eB.sample(1)
+        try:
+            eB.chains()
+            b.append(0)
+        except qcb.QcbError as err:
+            assert err.code == qcb.EBUDGET
+            b.append(_stopped(err))
+    assert np.array_equal(gA.get_links(), gB.get_links())
+    assert max(b) <= nA <= b[0] + b[1]
+    assert nA > b[1], (nA, b)         # stops of the first sample are not forgotten
+    for x in (eA, eB, gA, gB):
+        x.close()
+
+
+def test_lattice_outlives_its_chains(ctx):
+    """qcb_lattice_destroy refuses while event chains use the lattice; the
+    Python layer closes the chains first."""
+    g = qcb.Lattice(ctx, (8, 8, 8, 8))
+    e = qcb.Ecmc(g, qcb.ecmc_params(seed=1))
+    assert qcb.lib().qcb_lattice_destroy(g.h) == qcb.ESTATE
+    e.sample(1)
+    g.close()                         # closes e, then the lattice
+    assert not e.h and not g.h
+    # a context closes chains before lattices whatever the order of its children
+    c2 = qcb.Context(0)
+    g2 = qcb.Lattice(c2, (8, 8, 8, 8))
+    e2 = qcb.Ecmc(g2, qcb.ecmc_params(seed=1))
+    c2.close()
+    assert not e2.h and not g2.h

@@ -201,7 +201,7 @@ def test_probe_cm_hit(ctx):
     skipped = 0
     for t in range(n):
         want = qo.m3(U[t].copy())
-        qo.lib().qo_cm_hit_injected(want, qo.dp(qo.m3(A[t])), 6.0, qo.dp(np.ascontiguousarray(u[t])), K)
+        qo.lib().qo_cm_hit_injected(qo.dp(want), qo.dp(qo.m3(A[t])), 6.0, qo.dp(np.ascontiguousarray(u[t])), K)
         want = qo.c3(want)
         assert np.abs(out64[t] - want).max() < 1e-14
         _, info = cm.cm_hit(U[t], A[t], 6.0, u[t], K)
