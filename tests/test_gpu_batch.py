@@ -118,3 +118,54 @@ def test_batch_flow_steps_equal_alone(ctx, prec):
         assert b.lattice(i).flow_qe() == g.flow_qe()
         g.close()
     b.close()
+
+
+def _config4():
+    """64 lattices, 16 each of L = 8, 10, 12, 16 (L^4, periodic), beta ~ U[5.8, 6.2]
+    from seed 3, one Philox key per lattice."""
+    dims = [(L,) * 4 for L in (8, 10, 12, 16) for _ in range(16)]
+    betas = np.random.default_rng(3).uniform(5.8, 6.2, size=64)
+    seeds = [3000 + i for i in range(64)]
+    return dims, betas, seeds
+
+
+def test_config4_workload_equals_alone(ctx):
+    """The 64-lattice workload as a user runs it: autotuned schedule, 100
+    sweeps replayed from a hipGraph of 10, then a 50-step flow measured at
+    every step.  Two lattices of each size, run alone with the same schedule,
+    give bitwise the same links and (t, Q, E) series; with the default order
+    the batch also equals the single-lattice entry points."""
+    dims, betas, seeds = _config4()
+    b = qcb.Batch(ctx, dims, qcb.F32, betas, seeds)
+    b.hot()
+    report = b.autotune(2, 1)
+    order, grid = b.get_schedule()
+    assert report.count("ms_per_sweep") >= 2
+    b.graph_capture(2, 10, first_sweep=0)
+    b.graph_launch(10)
+    flows = b.flow(0.02, 50)
+    assert flows.shape == (64, 51, 3)
+    for i in (0, 5, 16, 27, 32, 40, 48, 63):
+        a = qcb.Batch(ctx, [dims[i]], qcb.F32, [betas[i]], [seeds[i]])
+        a.set_schedule(order, 0)
+        a.hot()
+        a.hb_sweeps(2, 100)
+        assert np.array_equal(b.lattice(i).get_links(), a.lattice(0).get_links()), i
+        fa = a.flow(0.02, 50)
+        assert np.array_equal(flows[i], fa[0]), i
+        a.close()
+    b.close()
+    # default (mu-major) order: batch == qcb_hb_sweeps + qcb_topo_charge_flowed on one lattice
+    b = qcb.Batch(ctx, dims, qcb.F32, betas, seeds)
+    b.hot()
+    b.hb_sweeps(2, 100)
+    flows = b.flow(0.02, 50)
+    for i in (3, 21, 37, 52):
+        g = qcb.Lattice(ctx, dims[i], qcb.F32)
+        g.hot(seeds[i])
+        g.hb_sweeps(betas[i], 2, 100, seed=seeds[i], sweep_idx=0)
+        assert np.array_equal(b.lattice(i).get_links(), g.get_links()), i
+        t, fin = g.topo_charge_flowed(0.02, 50, 1, final=True)
+        assert np.array_equal(flows[i, :50], t) and np.array_equal(flows[i, 50], fin), i
+        g.close()
+    b.close()

@@ -63,3 +63,22 @@ def test_service_equals_single_lattice_path(ctx):
     t, fin = lat.topo_charge_flowed(EPS, 4, 1, final=True)
     assert np.array_equal(res[:4], t)
     assert np.array_equal(res[4], fin)
+
+
+def test_config5_sizes_alone_equals_mixed(ctx):
+    """Requests of the serving workload's sizes, L^3 x 2L with L in {12, 16, 24},
+    flowed to t_max in [0.5, 2.0]: the 24^3 x 48 request's series is bitwise
+    the same alone and in a mix that arrives while it is resident."""
+    target = (9, (24, 24, 24, 48), 6.0, 123, 0.5, 0)
+    alone = run(qcb.Service(ctx, qcb.F32, 2 << 30, 2, N_THERM, 4, EPS), [target])[9][0]
+    assert alone.shape == (26, 3)
+    assert np.allclose(alone[:, 0], EPS * np.arange(26))
+    mix = [(1, (12, 12, 12, 24), 5.9, 1, 2.0, 0), (2, (16, 16, 16, 32), 6.1, 2, 1.0, 2),
+           (9, (24, 24, 24, 48), 6.0, 123, 0.5, 3), (3, (12, 12, 12, 24), 6.0, 3, 0.5, 4),
+           (4, (24, 24, 24, 48), 5.9, 4, 0.8, 5), (5, (16, 16, 16, 32), 6.0, 5, 1.5, 30)]
+    # two 24^3 x 48 requests (~490 MB each in FP32) cannot both fit: the second queues
+    got = run(qcb.Service(ctx, qcb.F32, 900 << 20, 2, N_THERM, 4, EPS), mix)
+    assert set(got) == {1, 2, 3, 4, 5, 9}
+    assert np.array_equal(got[9][0], alone)
+    assert got[1][0].shape == (101, 3)
+    assert got[4][1][1] >= got[9][1][2]          # admitted once request 9's slice was freed
