@@ -16,11 +16,24 @@
 #include <vector>
 
 #include "../../include/qcb.h"
-#include "hb_io.hpp"
 
 namespace qcb_mirror {
 
-using qcb_host::HbParams;
+// HbParams (src/io/params.h:17-23), with the reference defaults.
+struct HbParams {
+    double beta = 6.0;
+    int N_samples = 10;
+    int N_hits = 1;
+    int N_sweeps = 1;
+    int N_therm = 100;
+};
+
+// printf("%.*f"): the reference's std::fixed / setprecision text of a value
+inline std::string fixed_text(double v, int decimals) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.*f", decimals, v);
+    return buf;
+}
 
 inline qcb_ctx*& device_ctx() {
     static qcb_ctx* c = nullptr;
@@ -184,8 +197,8 @@ inline std::vector<double> topo_charge_flowed(GaugeField& field, const GeometryC
     if (topo.rank == 0)
         for (int s = 0; s < N_steps_gf; ++s) {
             double t = tQE[3 * s], q = tQE[3 * s + 1], e = tQE[3 * s + 2];
-            std::printf("n = %d, t = %s, Q = %s, t\xc2\xb2" "E = %s\n", s, qcb_host::io::format_double(t, 2).c_str(),
-                        qcb_host::io::format_double(q, 6).c_str(), qcb_host::io::format_double(t * t * e, 6).c_str());
+            std::printf("n = %d, t = %s, Q = %s, t\xc2\xb2" "E = %s\n", s, fixed_text(t, 2).c_str(),
+                        fixed_text(q, 6).c_str(), fixed_text(t * t * e, 6).c_str());
         }
     std::fflush(stdout);
     return tQE;

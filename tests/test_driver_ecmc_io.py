@@ -88,7 +88,7 @@ def test_defaults_and_missing_file(tmp_path):
     assert "Thermalization shifts : 100" in out and "Number of shifts : 2" in out
     assert "N_samples=1 N_switch_eo=3 cold_start=true" in out
     r = run(["--print-params", str(tmp_path / "nope.txt")])
-    assert r.returncode == 1 and "Impossible d'ouvrir" in r.stderr
+    assert r.returncode == 1 and "Error reading input : cannot open input file" in r.stderr
 
 
 def rank_block(r, n):

@@ -83,7 +83,7 @@ def test_defaults_and_cold_start_literal(tmp_path):
 
 def test_missing_file(tmp_path):
     r = subprocess.run([DRIVER, "--print-params", str(tmp_path / "nope.txt")], capture_output=True, text=True)
-    assert r.returncode == 1 and "Can't open file" in r.stderr
+    assert r.returncode == 1 and "Error reading input : cannot open input file" in r.stderr
 
 
 def test_precision_keys(tmp_path):
