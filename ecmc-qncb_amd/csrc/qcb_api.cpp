@@ -39,6 +39,13 @@ struct DevTmp {
 };
 }  // namespace
 
+// Static device scratch of the entry points that allocate nothing: 1 MiB, the
+// (t, Q, E) triples of up to kScratchDoubles / 3 flow measurements.  Each
+// device holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (ctx->scratch).
+constexpr size_t kScratchDoubles = (size_t)1 << 17;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 // ------------------------------------------------------------ context -----
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...) {
@@ -63,7 +70,11 @@ extern "C" int qcb_ctx_create(int device, qcb_ctx** out) {
     qcb_ctx* c = new qcb_ctx();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipGetSymbolAddress((void**)&c->scratch, HIP_SYMBOL(scratch)) != hipSuccess || !c->scratch) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->ev0) (void)hipEventDestroy(c->ev0);
+        if (c->ev1) (void)hipEventDestroy(c->ev1);
         delete c;
         return QCB_EDEVICE;
     }
@@ -524,6 +535,45 @@ extern "C" int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_
         final_tQE[1] = h[2 * n_steps_gf];
         final_tQE[2] = h[2 * n_steps_gf + 1];
     }
+    return QCB_OK;
+}
+
+// The same measurement schedule with nothing allocated: the flow buffers
+// must exist already (any earlier flow call on the lattice creates them) and
+// the per-measurement results are staged in the static scratch array.
+extern "C" int qcb_topo_charge_flowed_noalloc(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk, double* tQE,
+                                              double* final_tQE) {
+    if (!lat || n_steps_gf < 0 || n_rk < 0 || (n_steps_gf > 0 && !tQE)) return QCB_EINVAL;
+    qcb_ctx* ctx = lat->ctx;
+    const size_t nmeas = (size_t)n_steps_gf + 1;
+    if (3 * nmeas > kScratchDoubles)
+        return qcb_set_error(ctx, QCB_EINVAL, "at most %zu measurements per call", kScratchDoubles / 3 - 1);
+    if (!lat->F[0])
+        return qcb_set_error(ctx, QCB_ESTATE, "the lattice has no flow buffers yet: one flow call (e.g. "
+                                              "qcb_flow_copy) creates them");
+    int rc = flow_begin(lat);   // buffers present: copies the links, allocates nothing
+    if (rc) return rc;
+    double* d = ctx->scratch;   // measurement s at d[3s .. 3s+2]: Q, E/V written at 3s+1, 3s+2
+    for (int s = 0; s < n_steps_gf; ++s) {
+        if ((rc = enqueue_clover(lat, 1, d + 3 * s + 1))) return rc;
+        for (int r = 0; r < n_rk; ++r)
+            if ((rc = enqueue_flow_step(lat, eps))) return rc;
+    }
+    if (final_tQE && (rc = enqueue_clover(lat, 1, d + 3 * n_steps_gf + 1))) return rc;
+    if (n_steps_gf > 0)
+        QCB_HIP(ctx, hipMemcpyAsync(tQE, d, 3 * (size_t)n_steps_gf * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    if (final_tQE)
+        QCB_HIP(ctx, hipMemcpyAsync(final_tQE, d + 3 * n_steps_gf, 3 * sizeof(double), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    // flow times accumulated as the reference does (t += eps per RK3 step)
+    double t = 0.0;
+    for (int s = 0; s < n_steps_gf; ++s) {
+        tQE[3 * s] = t;
+        for (int r = 0; r < n_rk; ++r) t += eps;
+    }
+    if (final_tQE) final_tQE[0] = t;
     return QCB_OK;
 }
 

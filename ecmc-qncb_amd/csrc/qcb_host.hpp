@@ -15,6 +15,7 @@ struct qcb_ctx {
     std::string err;
     int n_live = 0;   // lattices and batches created on this context and not yet destroyed
     uint32_t kp_cap = 1024;   // Kennedy-Pendleton attempts per draw for Philox heatbath sweeps
+    double* scratch = nullptr;   // this device's address of the library's static scratch array
 };
 
 namespace qcb {

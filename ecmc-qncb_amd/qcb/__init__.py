@@ -85,6 +85,7 @@ SIGNATURES = [
     ("qcb_flow_plaquette", C.c_int, [_VP, _D]),
     ("qcb_flow_get_links", C.c_int, [_VP, _D]),
     ("qcb_topo_charge_flowed", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
+    ("qcb_topo_charge_flowed_noalloc", C.c_int, [_VP, C.c_double, C.c_int, C.c_int, _D, _D]),
     ("qcb_batch_create", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int), C.c_int, _D, C.POINTER(C.c_uint64),
                                     C.POINTER(_VP)]),
     ("qcb_batch_destroy", C.c_int, [_VP]),
@@ -317,6 +318,15 @@ class Lattice:
 
     def ildg_load(self, path: str):
         self._c(lib().qcb_ildg_load(self.h, str(path).encode()))
+
+    def topo_charge_flowed_noalloc(self, eps: float, n_steps_gf: int, n_rk: int, final: bool = False):
+        """qcb_topo_charge_flowed_noalloc: the same series, nothing allocated on the device."""
+        tqe = np.zeros(3 * max(n_steps_gf, 1))
+        fin = np.zeros(3)
+        self._c(lib().qcb_topo_charge_flowed_noalloc(self.h, eps, n_steps_gf, n_rk, _dp(tqe),
+                                                     _dp(fin) if final else None))
+        t = tqe[: 3 * n_steps_gf].reshape(n_steps_gf, 3)
+        return (t, fin) if final else t
 
     def topo_charge_flowed(self, eps: float, n_steps_gf: int, n_rk: int, final: bool = False):
         tqe = np.zeros(3 * max(n_steps_gf, 1))

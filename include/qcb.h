@@ -150,6 +150,16 @@ int qcb_flow_get_links(qcb_lattice* lat, double* aos);
  * after the last step (the reference never measures it). */
 int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk_steps,
                            double* tQE, double* final_tQE);
+/* No counterpart in the reference: qcb_topo_charge_flowed for callers that
+ * cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  Same results; it calls no hipMalloc/hipFree.  The lattice's
+ * flow buffers must exist (any earlier flow call creates them; QCB_ESTATE
+ * otherwise) and the measurement results are staged in a static 1 MiB device
+ * array that the library holds per GPU until the process ends, so
+ * n_steps_gf + 1 <= 43690 (QCB_EINVAL beyond).  Every call on one GPU shares
+ * that array: calls that overlap on a GPU (other threads or contexts) race. */
+int qcb_topo_charge_flowed_noalloc(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk_steps,
+                                   double* tQE, double* final_tQE);
 
 /* ---- batches of independent lattices ------------------------------------ */
 /* n lattices of uneven extents (dims[4*i..4*i+3]), each with its own beta and
