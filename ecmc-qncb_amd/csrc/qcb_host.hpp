@@ -46,6 +46,8 @@ struct qcb_lattice {
     uint32_t kp_cap = 1024;      // attempt cap in the uploaded descriptor
     unsigned long long* kp_exh = nullptr;   // where the heatbath counts exhausted draws (d_counter by default)
     int n_users = 0;             // event-chain handles reading this lattice (it must outlive them)
+    uint32_t flow_limit = 0xFFFFFFFFu;   // RK3 steps after flow_start (the service's per-request t_max)
+    double* qe_out = nullptr;    // per-step measurement slots (the service's result area)
 };
 
 int qcb_set_error(qcb_ctx* ctx, int code, const char* fmt, ...);
@@ -129,6 +131,8 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
     d.morton = l->morton;
     d.kp_cap = l->kp_cap;
     d.kp_exh = l->kp_exh;
+    d.flow_limit = l->flow_limit;
+    d.qe_out = l->qe_out;
     d.tbits[0] = 0;
     for (int k = 1; k < 4; ++k) {
         int b = 0;

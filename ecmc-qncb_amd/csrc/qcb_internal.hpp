@@ -23,14 +23,15 @@ struct Launch {
 enum HbMode { kHbPhilox = 0, kHbInjected = 1 };
 
 template <class T> void launch_cold(const Launch& g);
-template <class T> void launch_hot(const Launch& g);   // key from descriptor
+// gate: per-lattice cadence (Gate{} = every lattice of the launch)
+template <class T> void launch_hot(const Launch& g, Gate gate = Gate{});   // key from descriptor
 template <class T> void launch_aos_to_soa(const Launch& g, const double* aos, T* dst_field);
 // ILDG staging of links [lo, lo + n) in file order (single-lattice launches)
 template <class T> void launch_soa_to_be(const Launch& g, const T* src, uint64_t* out, uint64_t lo, uint32_t n);
 template <class T> void launch_be_to_soa(const Launch& g, const uint64_t* in, T* dst, uint64_t lo, uint32_t n);
 template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
 template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
-template <class T> void launch_flow_init(const Launch& g);   // F[0] <- U for every lattice
+template <class T> void launch_flow_init(const Launch& g, Gate gate = Gate{});   // F[0] <- U
 // which = 0: the links; 1: each lattice's current flow field at flow step gstep
 template <class T> void launch_plaquette(const Launch& g, double* partial, int which = 0, uint32_t gstep = 0);
 template <class T> void launch_clover(const Launch& g, int which_field, uint32_t gstep, double* partial_q, double* partial_e);
@@ -42,7 +43,7 @@ template <class T> void launch_hb_half(const Launch& g, int mu, int parity, int 
                                        int mode, const double* inj, int K, const uint32_t* sweep_base = nullptr,
                                        uint32_t grid = 0);
 void launch_bump(uint32_t* counter, uint32_t n, hipStream_t st);
-template <class T> void launch_reunit(const Launch& g);
+template <class T> void launch_reunit(const Launch& g, Gate gate = Gate{});
 template <class T> void launch_translate(const Launch& g, const int s[4], T* dst_field);
 // One RK3 stage on every lattice of the batch.  Field roles follow the
 // per-lattice step parity (gstep - flow_start) & 1; which_src = 0 reads U for
@@ -54,7 +55,8 @@ template <class T> void launch_math_probe(int op, int n, const double* in, doubl
 // Fixed-order per-lattice reductions of per-block partials.
 // plaq: out[l*ostride] = sum / (6 V);  qe: out[l*ostride] = Q, out[l*ostride+1] = E / V
 template <class T> void launch_finalize_plaq(const Launch& g, const double* partial, double* out, int ostride);
+// qe: a lattice with qe_out set writes at its own flow step gstep - flow_start instead
 template <class T> void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, double* out,
-                                           int ostride);
+                                           int ostride, uint32_t gstep = 0);
 
 }  // namespace qcb

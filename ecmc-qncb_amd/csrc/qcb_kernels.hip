@@ -87,10 +87,16 @@ __global__ __launch_bounds__(kBlock) void k_cold(const LatDev<T>* __restrict__ l
 }
 
 template <class T>
+__device__ __forceinline__ bool gate_open(const Gate& g, const LatDev<T>& L) {
+    return g.mod == 0u || (g.sweep + L.sweep_off) % g.mod == g.res;
+}
+
+template <class T>
 __global__ __launch_bounds__(kBlock) void k_hot(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                int n_lat, uint32_t nblocks) {
+                                                int n_lat, uint32_t nblocks, Gate gate) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
+    if (!gate_open(gate, L)) return;
     const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
     const uint32_t hs = threadIdx.x / kLinkSites;
     if (i >= L.Vh) return;
@@ -200,9 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restri
 // Flow start for every lattice of a batch: F[0] <- U (GradientFlow::copy).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_flow_init(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                      int n_lat, uint32_t nblocks) {
+                                                      int n_lat, uint32_t nblocks, Gate gate) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
+    if (!gate_open(gate, L)) return;
     const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
     const uint32_t hs = threadIdx.x / kLinkSites;
     if (i >= L.Vh) return;
@@ -231,9 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_translate(const LatDev<T>* __restric
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_reunit(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                   int n_lat, uint32_t nblocks) {
+                                                   int n_lat, uint32_t nblocks, Gate gate) {
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
+    if (!gate_open(gate, L)) return;
     const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
     const uint32_t hs = threadIdx.x / kLinkSites;
     if (i >= L.Vh) return;
@@ -413,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize_plaq(const LatDev<T>* __res
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_finalize_qe(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                         const double* __restrict__ pq, const double* __restrict__ pe,
-                                                        double* __restrict__ out, int ostride) {
+                                                        double* __restrict__ out, int ostride, uint32_t gstep) {
     __shared__ double lds[kBlock / 64];
     const int l = blockIdx.x;
     double s = 0.0, t = 0.0;
@@ -424,8 +432,9 @@ __global__ __launch_bounds__(kBlock) void k_finalize_qe(const LatDev<T>* __restr
     double rq = block_sum(s, lds);
     double re = block_sum(t, lds);
     if (threadIdx.x == 0) {
-        out[(size_t)l * ostride] = rq;
-        out[(size_t)l * ostride + 1] = re / (2.0 * (double)lats[l].Vh);
+        double* o = lats[l].qe_out ? lats[l].qe_out + 2 * (size_t)(gstep - lats[l].flow_start) : out + (size_t)l * ostride;
+        o[0] = rq;
+        o[1] = re / (2.0 * (double)lats[l].Vh);
     }
 }
 
@@ -798,6 +807,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     __shared__ T lds[6 * 2 * 18 * kLinkSites];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
+    if (gstep - L.flow_start >= L.flow_limit) return;   // the whole block: its lattice has finished flowing
     const uint32_t s = threadIdx.x & (kLinkSites - 1);
     const uint32_t q = bc.local * kLinkSites + s;
     const uint32_t hs = threadIdx.x / kLinkSites;
@@ -899,8 +909,8 @@ void launch_cold(const Launch& g) {
     hipLaunchKernelGGL(k_cold<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
 }
 template <class T>
-void launch_hot(const Launch& g) {
-    hipLaunchKernelGGL(k_hot<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+void launch_hot(const Launch& g, Gate gate) {
+    hipLaunchKernelGGL(k_hot<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gate);
 }
 template <class T>
 void launch_aos_to_soa(const Launch& g, const double* aos, T* dst) {
@@ -925,8 +935,8 @@ void launch_copy_field(const Launch& g, const T* src, T* dst) {
     hipLaunchKernelGGL(k_copy_field<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, dst);
 }
 template <class T>
-void launch_flow_init(const Launch& g) {
-    hipLaunchKernelGGL(k_flow_init<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+void launch_flow_init(const Launch& g, Gate gate) {
+    hipLaunchKernelGGL(k_flow_init<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gate);
 }
 template <class T>
 void launch_plaquette(const Launch& g, double* partial, int which, uint32_t gstep) {
@@ -942,8 +952,8 @@ void launch_staples(const Launch& g, double* out) {
     hipLaunchKernelGGL(k_staples<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), out);
 }
 template <class T>
-void launch_reunit(const Launch& g) {
-    hipLaunchKernelGGL(k_reunit<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g));
+void launch_reunit(const Launch& g, Gate gate) {
+    hipLaunchKernelGGL(k_reunit<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gate);
 }
 template <class T>
 void launch_translate(const Launch& g, const int s[4], T* dst) {
@@ -999,31 +1009,32 @@ void launch_finalize_plaq(const Launch& g, const double* partial, double* out, i
                        static_cast<const LatDev<T>*>(g.lats), g.blk, partial, out, ostride);
 }
 template <class T>
-void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, double* out, int ostride) {
+void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, double* out, int ostride,
+                        uint32_t gstep) {
     hipLaunchKernelGGL(k_finalize_qe<T>, dim3(g.n_lat), dim3(kBlock), 0, g.stream,
-                       static_cast<const LatDev<T>*>(g.lats), g.blk, pq, pe, out, ostride);
+                       static_cast<const LatDev<T>*>(g.lats), g.blk, pq, pe, out, ostride, gstep);
 }
 
 #define QCB_INSTANTIATE(T)                                                                                     \
     template void launch_cold<T>(const Launch&);                                                              \
-    template void launch_hot<T>(const Launch&);                                                               \
+    template void launch_hot<T>(const Launch&, Gate);                                                         \
     template void launch_aos_to_soa<T>(const Launch&, const double*, T*);                                     \
     template void launch_soa_to_aos<T>(const Launch&, const T*, double*);                                     \
     template void launch_copy_field<T>(const Launch&, const T*, T*);                                          \
     template void launch_soa_to_be<T>(const Launch&, const T*, uint64_t*, uint64_t, uint32_t);                 \
     template void launch_be_to_soa<T>(const Launch&, const uint64_t*, T*, uint64_t, uint32_t);                 \
-    template void launch_flow_init<T>(const Launch&);                                                         \
+    template void launch_flow_init<T>(const Launch&, Gate);                                                   \
     template void launch_plaquette<T>(const Launch&, double*, int, uint32_t);                                                \
     template void launch_clover<T>(const Launch&, int, uint32_t, double*, double*);                           \
     template void launch_staples<T>(const Launch&, double*);                                                       \
-    template void launch_reunit<T>(const Launch&);                                                            \
+    template void launch_reunit<T>(const Launch&, Gate);                                                      \
     template void launch_translate<T>(const Launch&, const int*, T*);                                         \
     template void launch_hb_half<T>(const Launch&, int, int, int, uint32_t, int, const double*, int,           \
                                     const uint32_t*, uint32_t);                                               \
     template void launch_flow_stage<T>(const Launch&, int, uint32_t, double);                                 \
     template void launch_math_probe<T>(int, int, const double*, double, int, double*, hipStream_t);           \
     template void launch_finalize_plaq<T>(const Launch&, const double*, double*, int);                        \
-    template void launch_finalize_qe<T>(const Launch&, const double*, const double*, double*, int);
+    template void launch_finalize_qe<T>(const Launch&, const double*, const double*, double*, int, uint32_t);
 
 QCB_INSTANTIATE(float)
 QCB_INSTANTIATE(double)

@@ -51,6 +51,15 @@ struct LatDev {
     int tbits[4];         // log2 NT[d] when morton
     uint32_t kp_cap;      // Kennedy-Pendleton attempts per subgroup draw in Philox mode
     unsigned long long* kp_exh;  // device count of draws that ran out of attempts (skipped updates)
+    uint32_t flow_limit;  // RK3 steps this lattice takes after flow_start (later stage launches skip it)
+    double* qe_out;       // if set, a measurement at flow step s writes (Q, E/V) to qe_out[2s], qe_out[2s+1]
+};
+
+// A per-lattice cadence for kernels launched over a set of lattices that are
+// each at their own point of a schedule: lattice L takes part iff mod == 0 or
+// (sweep + L.sweep_off) % mod == res, i.e. its own sweep count hits the cadence.
+struct Gate {
+    uint32_t sweep = 0, mod = 0, res = 0;
 };
 
 // Threads per block of each kernel family and the work item of a thread:

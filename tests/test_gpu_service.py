@@ -82,3 +82,20 @@ def test_config5_sizes_alone_equals_mixed(ctx):
     assert np.array_equal(got[9][0], alone)
     assert got[1][0].shape == (101, 3)
     assert got[4][1][1] >= got[9][1][2]          # admitted once request 9's slice was freed
+
+
+def test_edge_schedules_and_many_requests(ctx):
+    """No thermalisation and t_max = 0: a request's first iteration both hot-starts
+    and prepares the flow, and a zero-length flow is one measurement.  Many
+    small requests at once grow the descriptor sets past their first size."""
+    svc = qcb.Service(ctx, qcb.F64, 1 << 30, 2, 0, 0, EPS)
+    reqs = [(i, (4, 4, 4, 8), 6.0, 50 + i, [0.0, 0.04, 0.1][i % 3], i % 5) for i in range(90)]
+    got = run(svc, reqs)
+    assert set(got) == set(range(90))
+    for i in (0, 1, 2, 47):
+        lat = qcb.Lattice(ctx, (4, 4, 4, 8), qcb.F64)
+        lat.hot(50 + i)
+        n = int(round(reqs[i][4] / EPS))
+        t, fin = lat.topo_charge_flowed(EPS, n, 1, final=True)
+        assert np.array_equal(got[i][0][:n], t) and np.array_equal(got[i][0][n], fin), i
+        lat.close()
