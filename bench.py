@@ -128,6 +128,7 @@ def batch64(qcb, ctx, sweeps=20, flow_steps=50):
     b.hot()
     report = b.autotune(2, 2)
     order, grid = b.get_schedule()
+    n_streams, tiles = b.get_layout()
     b.graph_capture(2, 10, first_sweep=0)
     b.graph_launch(1)
     ctx.sync()
@@ -149,7 +150,8 @@ def batch64(qcb, ctx, sweeps=20, flow_steps=50):
     b.close()
     flow_rate = V * flow_steps / (fms * 1e-3)
     return dict(lattices=64, sites=V, link_updates_per_s=4 * V * 10 * reps / (ms * 1e-3),
-                ms_per_sweep=ms / (10 * reps), schedule={"order": order, "grid": grid},
+                ms_per_sweep=ms / (10 * reps),
+                schedule={"order": order, "grid": grid, "streams": n_streams, "tile_caps": list(tiles)},
                 autotune=report.strip().split("\n"),
                 flow_site_steps_per_s_incl_measure=V * flow_steps / dt,
                 flow_site_steps_per_s=flow_rate, flow_achieved_gbs=WF_BYTES_PER_SITE * 4 * flow_rate / 1e9,

@@ -1,6 +1,13 @@
 // qcb_batch.cpp -- many independent lattices (uneven sizes, own beta and
 // Philox key each) advanced by one launch per (mu, parity) half-sweep, per RK3
 // stage and per reduction; a tuned launch schedule; hipGraph replay of sweeps.
+//
+// The heatbath can split the lattices into groups, each with its own
+// descriptor table and stream: a group's half-sweeps depend only on its own
+// earlier half-sweeps, so while one group's launch drains its last blocks
+// the next group's launch fills the idle compute units.  The groups fork from
+// the context stream and join back into it through events, which a graph
+// capture records as parallel branches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +41,17 @@ struct qcb_batch {
     // launch schedule (autotuned)
     int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};   // half-sweep order, hs = 2*mu + parity
     uint32_t hb_grid = 0;                       // 0: one block per tile; else persistent grid
+    // stream groups of the heatbath: lattices grp_idx[g], their descriptors and
+    // block prefixes on the device, and the stream of each group (group 0 runs
+    // on the context stream)
+    int n_groups = 1;
+    std::vector<std::vector<int>> grp_idx;
+    std::vector<void*> grp_desc;
+    std::vector<uint32_t*> grp_blk;
+    std::vector<uint32_t> grp_nb;
+    std::vector<hipStream_t> grp_stream;
+    std::vector<hipEvent_t> grp_event;   // [0] fork, [g] join of group g
+    int tile_caps[3] = {8, 4, 4};        // work-order tile caps in (y, z, t)
     // captured sweeps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -42,8 +60,24 @@ struct qcb_batch {
 
 namespace {
 
+template <class T>
+int upload_groups(qcb_batch* b) {
+    qcb_ctx* c = b->ctx;
+    for (size_t g = 0; g < b->grp_idx.size(); ++g) {
+        std::vector<LatDev<T>> d;
+        for (int i : b->grp_idx[g]) d.push_back(make_desc<T>(b->lats[i]));
+        QCB_HIP(c, hipMemcpyAsync(b->grp_desc[g], d.data(), d.size() * sizeof(d[0]), hipMemcpyHostToDevice, c->stream));
+        QCB_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return QCB_OK;
+}
+
 int batch_upload(qcb_batch* b) {
     qcb_ctx* c = b->ctx;
+    if (b->n_groups > 1) {
+        const int rc = b->prec == QCB_F64 ? upload_groups<double>(b) : upload_groups<float>(b);
+        if (rc) return rc;
+    }
     if (b->prec == QCB_F64) {
         std::vector<LatDev<double>> d(b->n);
         for (int i = 0; i < b->n; ++i) d[i] = make_desc<double>(b->lats[i]);
@@ -68,7 +102,21 @@ Launch batch_launch(const qcb_batch* b, int fam) {
     return g;
 }
 
+void free_groups(qcb_batch* b) {
+    for (void* d : b->grp_desc) (void)hipFree(d);
+    for (uint32_t* d : b->grp_blk) (void)hipFree(d);
+    for (size_t g = 1; g < b->grp_stream.size(); ++g) (void)hipStreamDestroy(b->grp_stream[g]);
+    for (hipEvent_t e : b->grp_event) (void)hipEventDestroy(e);
+    b->grp_desc.clear();
+    b->grp_blk.clear();
+    b->grp_stream.clear();
+    b->grp_event.clear();
+    b->grp_nb.clear();
+    b->grp_idx.clear();
+}
+
 void batch_free(qcb_batch* b) {
+    free_groups(b);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (qcb_lattice* l : b->lats)
@@ -90,19 +138,101 @@ int ensure_meas(qcb_batch* b, size_t n) {
     return QCB_OK;
 }
 
-int enqueue_batch_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx, const uint32_t* base) {
-    Launch g = batch_launch(b, kFamHalf);
-    for (int s = 0; s < n_sweeps; ++s)
-        for (int k = 0; k < 8; ++k) {
-            const int mu = b->order[k] >> 1, p = b->order[k] & 1;
-            if (b->prec == QCB_F64)
-                launch_hb_half<double>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, base,
-                                       b->hb_grid);
-            else
-                launch_hb_half<float>(g, mu, p, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr, 0, base,
-                                      b->hb_grid);
+// Splits the lattices into n groups of about equal work (greedy by size) with
+// their own descriptor tables and streams.
+int set_groups(qcb_batch* b, int n) {
+    qcb_ctx* c = b->ctx;
+    QCB_HIP(c, hipStreamSynchronize(c->stream));
+    free_groups(b);
+    n = std::max(1, std::min(n, b->n));
+    b->n_groups = n;
+    if (n == 1) return QCB_OK;
+    std::vector<int> ids(b->n);
+    for (int i = 0; i < b->n; ++i) ids[i] = i;
+    std::stable_sort(ids.begin(), ids.end(), [&](int x, int y) { return b->lats[x]->Vh > b->lats[y]->Vh; });
+    b->grp_idx.assign(n, {});
+    std::vector<uint64_t> load(n, 0);
+    for (int i : ids) {
+        const int g = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->grp_idx[g].push_back(i);
+        load[g] += b->lats[i]->Vh;
+    }
+    for (auto& v : b->grp_idx) std::sort(v.begin(), v.end());
+    b->grp_desc.assign(n, nullptr);
+    b->grp_blk.assign(n, nullptr);
+    b->grp_nb.assign(n, 0);
+    b->grp_stream.assign(n, nullptr);
+    b->grp_event.assign(n, nullptr);
+    b->grp_stream[0] = c->stream;
+    for (int g = 0; g < n; ++g) {
+        const auto& idx = b->grp_idx[g];
+        QCB_HIP(c, hipMalloc(&b->grp_desc[g], idx.size() * sizeof(LatDev<double>)));
+        std::vector<uint32_t> pre(idx.size() + 1, 0);
+        for (size_t k = 0; k < idx.size(); ++k) pre[k + 1] = pre[k] + fam_blocks(b->lats[idx[k]]->Vh, kFamHalf);
+        b->grp_nb[g] = pre.back();
+        QCB_HIP(c, hipMalloc((void**)&b->grp_blk[g], pre.size() * sizeof(uint32_t)));
+        QCB_HIP(c, hipMemcpy(b->grp_blk[g], pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        if (g > 0) QCB_HIP(c, hipStreamCreateWithFlags(&b->grp_stream[g], hipStreamNonBlocking));
+        QCB_HIP(c, hipEventCreateWithFlags(&b->grp_event[g], hipEventDisableTiming));
+    }
+    return b->prec == QCB_F64 ? upload_groups<double>(b) : upload_groups<float>(b);
+}
+
+// Work-order tiles of every lattice from the caps (largest divisor of each
+// extent not above its cap; Z-order where the tile counts allow it).
+int apply_tiles(qcb_batch* b) {
+    for (qcb_lattice* l : b->lats) {
+        int t[4] = {0, divisor_le(l->L[1], b->tile_caps[0]), divisor_le(l->L[2], b->tile_caps[1]),
+                    divisor_le(l->L[3], b->tile_caps[2])};
+        int m = 1;
+        for (int d = 1; d < 4; ++d) {
+            const int nt = l->L[d] / t[d];
+            if (nt & (nt - 1)) m = 0;
         }
-    QCB_HIP(b->ctx, hipGetLastError());
+        for (int d = 1; d < 4; ++d) l->tile[d] = t[d];
+        l->morton = m;
+        const int rc = upload_desc(l);
+        if (rc) return rc;
+    }
+    return batch_upload(b);
+}
+
+template <class T>
+void launch_sweeps_on(const Launch& g, const qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx,
+                      const uint32_t* base) {
+    for (int s = 0; s < n_sweeps; ++s)
+        for (int k = 0; k < 8; ++k)
+            launch_hb_half<T>(g, b->order[k] >> 1, b->order[k] & 1, n_hits, sweep_idx + (uint32_t)s, kHbPhilox, nullptr,
+                              0, base, b->hb_grid);
+}
+
+int enqueue_batch_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_idx, const uint32_t* base) {
+    qcb_ctx* c = b->ctx;
+    if (b->n_groups <= 1) {
+        Launch g = batch_launch(b, kFamHalf);
+        if (b->prec == QCB_F64) launch_sweeps_on<double>(g, b, n_hits, n_sweeps, sweep_idx, base);
+        else launch_sweeps_on<float>(g, b, n_hits, n_sweeps, sweep_idx, base);
+        QCB_HIP(c, hipGetLastError());
+        return QCB_OK;
+    }
+    // fork the groups from the context stream, join them back into it
+    QCB_HIP(c, hipEventRecord(b->grp_event[0], c->stream));
+    for (int g = 1; g < b->n_groups; ++g) QCB_HIP(c, hipStreamWaitEvent(b->grp_stream[g], b->grp_event[0], 0));
+    for (int g = 0; g < b->n_groups; ++g) {
+        Launch gl;
+        gl.lats = b->grp_desc[g];
+        gl.blk = b->grp_blk[g];
+        gl.n_lat = (int)b->grp_idx[g].size();
+        gl.nblocks = b->grp_nb[g];
+        gl.stream = b->grp_stream[g];
+        if (b->prec == QCB_F64) launch_sweeps_on<double>(gl, b, n_hits, n_sweeps, sweep_idx, base);
+        else launch_sweeps_on<float>(gl, b, n_hits, n_sweeps, sweep_idx, base);
+    }
+    QCB_HIP(c, hipGetLastError());
+    for (int g = 1; g < b->n_groups; ++g) {
+        QCB_HIP(c, hipEventRecord(b->grp_event[g], b->grp_stream[g]));
+        QCB_HIP(c, hipStreamWaitEvent(c->stream, b->grp_event[g], 0));
+    }
     return QCB_OK;
 }
 
@@ -360,6 +490,23 @@ extern "C" int qcb_batch_get_schedule(qcb_batch* b, int order[8], uint32_t* hb_g
     return QCB_OK;
 }
 
+extern "C" int qcb_batch_set_layout(qcb_batch* b, int n_streams, const int tile_caps[3]) {
+    if (!b || !tile_caps || n_streams < 1 || n_streams > 8) return QCB_EINVAL;
+    for (int d = 0; d < 3; ++d)
+        if (tile_caps[d] < 1) return qcb_set_error(b->ctx, QCB_EINVAL, "tile caps must be >= 1");
+    std::memcpy(b->tile_caps, tile_caps, sizeof b->tile_caps);
+    int rc = apply_tiles(b);
+    if (rc) return rc;
+    return n_streams == b->n_groups ? QCB_OK : set_groups(b, n_streams);
+}
+
+extern "C" int qcb_batch_get_layout(qcb_batch* b, int* n_streams, int tile_caps[3]) {
+    if (!b || !n_streams || !tile_caps) return QCB_EINVAL;
+    *n_streams = b->n_groups;
+    std::memcpy(tile_caps, b->tile_caps, sizeof b->tile_caps);
+    return QCB_OK;
+}
+
 // Times candidate schedules (half-sweep order x persistent grid) on a scratch
 // copy of every lattice's links and keeps the fastest.  The links themselves
 // are not modified.  report (optional) receives one line per candidate.
@@ -386,33 +533,66 @@ extern "C" int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* repo
         uint32_t g = (uint32_t)(ncu * per) & ~7u;
         if (g && g < b->nb[kFamHalf]) grids.push_back(g);
     }
+    std::vector<int> groups = {1};
+    for (int n : {2, 4})
+        if (n <= b->n) groups.push_back(n);
+    const int tiles[5][3] = {{8, 4, 4}, {4, 4, 4}, {4, 2, 2}, {8, 8, 8}, {2, 2, 2}};
     std::string rep;
-    double best = 1e300;
-    int best_o = 0;
+    auto time_one = [&](float* ms) -> int {
+        int r = enqueue_batch_sweeps(b, n_hits, 1, 0x7fff0000u, nullptr);   // warm-up
+        if (r) return r;
+        QCB_HIP(c, hipEventRecord(c->ev0, c->stream));
+        if ((r = enqueue_batch_sweeps(b, n_hits, reps, 0x7fff0001u, nullptr))) return r;
+        QCB_HIP(c, hipEventRecord(c->ev1, c->stream));
+        QCB_HIP(c, hipEventSynchronize(c->ev1));
+        QCB_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+        *ms /= reps;
+        char line[192];
+        std::snprintf(line, sizeof line, "order=%s grid=%u streams=%d tiles=%dx%dx%d ms_per_sweep=%.4f\n",
+                      b->order[1] == 1 ? "mu-major" : "parity-major", b->hb_grid, b->n_groups, b->tile_caps[0],
+                      b->tile_caps[1], b->tile_caps[2], *ms);
+        rep += line;
+        return QCB_OK;
+    };
+    // 1: half-sweep order x persistent grid x stream groups at the current tiles
+    float best = 1e30f;
+    int best_o = 0, best_n = b->n_groups;
     uint32_t best_g = 0;
-    for (int o = 0; o < 2; ++o)
-        for (uint32_t g : grids) {
-            std::memcpy(b->order, orders[o], sizeof b->order);
-            b->hb_grid = g;
-            if ((rc = enqueue_batch_sweeps(b, n_hits, 1, 0x7fff0000u, nullptr))) return rc;   // warm-up
-            QCB_HIP(c, hipEventRecord(c->ev0, c->stream));
-            if ((rc = enqueue_batch_sweeps(b, n_hits, reps, 0x7fff0001u, nullptr))) return rc;
-            QCB_HIP(c, hipEventRecord(c->ev1, c->stream));
-            QCB_HIP(c, hipEventSynchronize(c->ev1));
-            float ms = 0.f;
-            QCB_HIP(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-            char line[160];
-            std::snprintf(line, sizeof line, "order=%s grid=%u ms_per_sweep=%.4f\n", o ? "parity-major" : "mu-major", g,
-                          ms / reps);
-            rep += line;
-            if (ms < best) {
-                best = ms;
-                best_o = o;
-                best_g = g;
+    for (int n : groups) {
+        if ((rc = set_groups(b, n))) return rc;
+        for (int o = 0; o < 2; ++o)
+            for (uint32_t g : grids) {
+                std::memcpy(b->order, orders[o], sizeof b->order);
+                b->hb_grid = g;
+                float ms = 0.f;
+                if ((rc = time_one(&ms))) return rc;
+                if (ms < best) {
+                    best = ms;
+                    best_o = o;
+                    best_g = g;
+                    best_n = n;
+                }
             }
-        }
+    }
     std::memcpy(b->order, orders[best_o], sizeof b->order);
     b->hb_grid = best_g;
+    if ((rc = set_groups(b, best_n))) return rc;
+    // 2: work-order tiles with that schedule
+    int best_t[3];
+    std::memcpy(best_t, b->tile_caps, sizeof best_t);
+    for (const auto& t : tiles) {
+        if (std::equal(t, t + 3, best_t)) continue;   // timed in step 1
+        std::memcpy(b->tile_caps, t, sizeof b->tile_caps);
+        if ((rc = apply_tiles(b))) return rc;
+        float ms = 0.f;
+        if ((rc = time_one(&ms))) return rc;
+        if (ms < best) {
+            best = ms;
+            std::memcpy(best_t, t, sizeof best_t);
+        }
+    }
+    std::memcpy(b->tile_caps, best_t, sizeof b->tile_caps);
+    if ((rc = apply_tiles(b))) return rc;
     for (qcb_lattice* l : b->lats) std::swap(l->U, l->F[0]);
     if ((rc = batch_upload(b))) return rc;
     if (report && report_len) {

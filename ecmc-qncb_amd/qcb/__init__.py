@@ -98,6 +98,8 @@ SIGNATURES = [
     ("qcb_batch_flow_steps", C.c_int, [_VP, C.c_double, C.c_int]),
     ("qcb_batch_set_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.c_uint32]),
     ("qcb_batch_get_schedule", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_uint32)]),
+    ("qcb_batch_set_layout", C.c_int, [_VP, C.c_int, C.POINTER(C.c_int)]),
+    ("qcb_batch_get_layout", C.c_int, [_VP, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("qcb_batch_autotune", C.c_int, [_VP, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     ("qcb_batch_graph_capture", C.c_int, [_VP, C.c_int, C.c_int, C.c_uint32]),
     ("qcb_batch_graph_launch", C.c_int, [_VP, C.c_int]),
@@ -417,6 +419,15 @@ class Batch:
         g = C.c_uint32()
         self._c(lib().qcb_batch_get_schedule(self.h, o, C.byref(g)))
         return list(o), g.value
+
+    def set_layout(self, n_streams=1, tile_caps=(8, 4, 4)):
+        self._c(lib().qcb_batch_set_layout(self.h, n_streams, (C.c_int * 3)(*tile_caps)))
+
+    def get_layout(self):
+        n = C.c_int()
+        t = (C.c_int * 3)()
+        self._c(lib().qcb_batch_get_layout(self.h, C.byref(n), t))
+        return n.value, tuple(t)
 
     def autotune(self, n_hits=2, reps=2) -> str:
         buf = C.create_string_buffer(4096)

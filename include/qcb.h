@@ -190,7 +190,16 @@ int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps);
  * (0 = one block per tile, else a multiple of 8 blocks striding over tiles). */
 int qcb_batch_set_schedule(qcb_batch* b, const int order[8], uint32_t hb_grid);
 int qcb_batch_get_schedule(qcb_batch* b, int order[8], uint32_t* hb_grid);
-/* Times candidate schedules on scratch copies and keeps the fastest. */
+/* The heatbath's launch layout: the lattices split into n_streams groups of
+ * about equal work, each on its own stream (a group's tail overlaps the next
+ * group's half-sweep), and work-order tiles of each lattice capped at
+ * tile_caps = {ty, tz, tt} (largest divisors of the extents not above them).
+ * Results are bitwise those of one stream; the tiles change only the last bits
+ * of the FP64 global sums (their block partition). */
+int qcb_batch_set_layout(qcb_batch* b, int n_streams, const int tile_caps[3]);
+int qcb_batch_get_layout(qcb_batch* b, int* n_streams, int tile_caps[3]);
+/* Times candidate schedules on scratch copies and keeps the fastest: half-sweep
+ * order x persistent grid x stream groups, then tile caps. */
 int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len);
 /* hipGraph of sweeps_per_graph sweeps; replays continue the Philox sweep index. */
 int qcb_batch_graph_capture(qcb_batch* b, int n_hits, int sweeps_per_graph, uint32_t first_sweep);

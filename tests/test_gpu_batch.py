@@ -140,7 +140,8 @@ def test_config4_workload_equals_alone(ctx):
     b.hot()
     report = b.autotune(2, 1)
     order, grid = b.get_schedule()
-    assert report.count("ms_per_sweep") >= 2
+    n_streams, tiles = b.get_layout()
+    assert "streams=4" in report and "tiles=2x2x2" in report
     b.graph_capture(2, 10, first_sweep=0)
     b.graph_launch(10)
     flows = b.flow(0.02, 50)
@@ -148,6 +149,7 @@ def test_config4_workload_equals_alone(ctx):
     for i in (0, 5, 16, 27, 32, 40, 48, 63):
         a = qcb.Batch(ctx, [dims[i]], qcb.F32, [betas[i]], [seeds[i]])
         a.set_schedule(order, 0)
+        a.set_layout(1, tiles)
         a.hot()
         a.hb_sweeps(2, 100)
         assert np.array_equal(b.lattice(i).get_links(), a.lattice(0).get_links()), i
@@ -169,3 +171,29 @@ def test_config4_workload_equals_alone(ctx):
         assert np.array_equal(flows[i, :50], t) and np.array_equal(flows[i, 50], fin), i
         g.close()
     b.close()
+
+
+@pytest.mark.parametrize("prec", [qcb.F32, qcb.F64])
+def test_stream_groups_same_chain(ctx, prec):
+    """Lattices split over 1, 2 or 4 streams, swept directly or from a graph
+    replay, follow bitwise the same chain; tile caps change no link."""
+    dims, betas, seeds = _config4()
+    sel = list(range(0, 64, 5))
+    dims, betas, seeds = [dims[i] for i in sel], [betas[i] for i in sel], [seeds[i] for i in sel]
+    ref = qcb.Batch(ctx, dims, prec, betas, seeds)
+    ref.hot()
+    ref.hb_sweeps(2, 4)
+    want = [ref.lattice(i).get_links() for i in range(len(dims))]
+    ref.close()
+    for n, tiles in ((2, (4, 2, 2)), (4, (2, 2, 2)), (4, (8, 4, 4))):
+        b = qcb.Batch(ctx, dims, prec, betas, seeds)
+        b.set_layout(n, tiles)
+        assert b.get_layout() == (n, tiles)
+        b.hot()
+        b.hb_sweeps(2, 2)
+        b.graph_capture(2, 1, first_sweep=2)
+        b.graph_launch(1)
+        b.graph_launch(1)
+        for i in range(len(dims)):
+            assert np.array_equal(b.lattice(i).get_links(), want[i]), (n, tiles, i)
+        b.close()
