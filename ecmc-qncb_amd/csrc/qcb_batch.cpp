@@ -37,7 +37,6 @@ struct qcb_batch {
     size_t meas_cap = 0;
     uint32_t* d_sweep = nullptr;     // device sweep counter for graph replay
     unsigned long long* d_counter = nullptr;   // n exhausted-draw counts, one per lattice
-    uint32_t gstep = 0;
     // launch schedule (autotuned)
     int order[8] = {0, 1, 2, 3, 4, 5, 6, 7};   // half-sweep order, hs = 2*mu + parity
     uint32_t hb_grid = 0;                       // 0: one block per tile; else persistent grid
@@ -52,6 +51,15 @@ struct qcb_batch {
     std::vector<hipStream_t> grp_stream;
     std::vector<hipEvent_t> grp_event;   // [0] fork, [g] join of group g
     int tile_caps[3] = {8, 4, 4};        // work-order tile caps in (y, z, t)
+    // the flow's launch sequence, captured at its first call and replayed while
+    // eps, the step count and every buffer address stay the same
+    struct FlowGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        double eps = 0.0;
+        int n_steps = -1;
+        std::vector<const void*> key;
+    } flow_graph[2];                     // [0] measured flow, [1] steps only
     // captured sweeps
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -115,8 +123,17 @@ void free_groups(qcb_batch* b) {
     b->grp_idx.clear();
 }
 
+void free_flow_graphs(qcb_batch* b) {
+    for (auto& fg : b->flow_graph) {
+        if (fg.exec) (void)hipGraphExecDestroy(fg.exec);
+        if (fg.graph) (void)hipGraphDestroy(fg.graph);
+        fg = qcb_batch::FlowGraph{};
+    }
+}
+
 void batch_free(qcb_batch* b) {
     free_groups(b);
+    free_flow_graphs(b);
     if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     for (qcb_lattice* l : b->lats)
@@ -252,6 +269,42 @@ int zero_counters(qcb_batch* b) {
     return QCB_OK;
 }
 
+// Every address a flow launch sequence bakes into its kernel arguments.
+std::vector<const void*> flow_key(const qcb_batch* b) {
+    std::vector<const void*> k = {b->d_meas, b->d_partial, b->d_desc, b->d_blk[0], b->d_blk[1], b->d_blk[2]};
+    for (const qcb_lattice* l : b->lats) {
+        k.push_back(l->U);
+        for (void* f : l->F) k.push_back(f);
+        k.push_back(l->d_desc);
+        k.push_back(l->d_blk[kFamLink]);
+    }
+    return k;
+}
+
+// Replays fg if it was captured for these arguments and addresses, else
+// captures `enqueue` on the context stream into it and launches it.
+template <class F>
+int run_flow_graph(qcb_batch* b, qcb_batch::FlowGraph& fg, double eps, int n_steps, F enqueue) {
+    qcb_ctx* c = b->ctx;
+    std::vector<const void*> key = flow_key(b);
+    if (!(fg.exec && fg.eps == eps && fg.n_steps == n_steps && fg.key == key)) {
+        if (fg.exec) QCB_HIP(c, hipGraphExecDestroy(fg.exec));
+        if (fg.graph) QCB_HIP(c, hipGraphDestroy(fg.graph));
+        fg = qcb_batch::FlowGraph{};
+        QCB_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        const hipError_t e = hipStreamEndCapture(c->stream, &fg.graph);
+        if (rc) return rc;
+        if (e != hipSuccess) return qcb_set_error(c, QCB_EDEVICE, "flow graph capture failed: %s", hipGetErrorString(e));
+        QCB_HIP(c, hipGraphInstantiate(&fg.exec, fg.graph, nullptr, nullptr, 0));
+        fg.eps = eps;
+        fg.n_steps = n_steps;
+        fg.key = key;
+    }
+    QCB_HIP(c, hipGraphLaunch(fg.exec, c->stream));
+    return QCB_OK;
+}
+
 }  // namespace
 
 extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, const double* betas,
@@ -372,7 +425,9 @@ extern "C" int qcb_batch_plaquette(qcb_batch* b, double* out) {
 
 // Flow every lattice of the batch from its links: measure (Q, E/V) at
 // t = 0, eps, ..., n_steps*eps (measure-then-step, plus the final state).
-// tQE[(i*(n_steps+1) + s)*3 + {0,1,2}] = (t, Q, E_mean) of lattice i.
+// tQE[(i*(n_steps+1) + s)*3 + {0,1,2}] = (t, Q, E_mean) of lattice i.  Every
+// call counts its flow steps from zero, so calls with the same eps and step
+// count issue the same launches: they replay one captured graph.
 extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE) {
     if (!b || n_steps < 0 || !tQE) return QCB_EINVAL;
     qcb_ctx* c = b->ctx;
@@ -381,44 +436,47 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
     if (rc) return rc;
     for (qcb_lattice* l : b->lats) {
         if ((rc = ensure_flow_buffers(l))) return rc;
-        l->flow_start = b->gstep;
+        l->flow_start = 0;
         l->flow_active = true;
     }
     if ((rc = batch_upload(b))) return rc;
-    Launch gl = batch_launch(b, kFamLink), gs = batch_launch(b, kFamSite);
-    double* pq = b->d_partial;
-    double* pe = b->d_partial + b->nb[kFamSite];
-    // copy links into the flow fields of every lattice: one launch per lattice
-    for (qcb_lattice* l : b->lats) {
-        Launch one = lat_launch(l, kFamLink);
-        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
-        else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
-    }
-    std::vector<double> tv(nm);
-    double t = 0.0;
-    for (int s = 0; s < nm; ++s) {
-        tv[s] = t;
-        if (b->prec == QCB_F64) {
-            launch_clover<double>(gs, 1, b->gstep, pq, pe);
-            launch_finalize_qe<double>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
-        } else {
-            launch_clover<float>(gs, 1, b->gstep, pq, pe);
-            launch_finalize_qe<float>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
+    const bool f64 = b->prec == QCB_F64;
+    auto enqueue = [&]() -> int {
+        Launch gl = batch_launch(b, kFamLink), gs = batch_launch(b, kFamSite);
+        double* pq = b->d_partial;
+        double* pe = b->d_partial + b->nb[kFamSite];
+        // copy links into the flow fields of every lattice: one launch per lattice
+        for (qcb_lattice* l : b->lats) {
+            Launch one = lat_launch(l, kFamLink);
+            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
+            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
         }
-        if (s == n_steps) break;
-        for (int st = 1; st <= 3; ++st) {
-            if (b->prec == QCB_F64) launch_flow_stage<double>(gl, st, b->gstep, eps);
-            else launch_flow_stage<float>(gl, st, b->gstep, eps);
+        for (int s = 0; s < nm; ++s) {
+            if (f64) {
+                launch_clover<double>(gs, 1, (uint32_t)s, pq, pe);
+                launch_finalize_qe<double>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
+            } else {
+                launch_clover<float>(gs, 1, (uint32_t)s, pq, pe);
+                launch_finalize_qe<float>(gs, pq, pe, b->d_meas + 2 * s, 2 * nm);
+            }
+            if (s == n_steps) break;
+            for (int st = 1; st <= 3; ++st) {
+                if (f64) launch_flow_stage<double>(gl, st, (uint32_t)s, eps);
+                else launch_flow_stage<float>(gl, st, (uint32_t)s, eps);
+            }
         }
-        b->gstep++;
-        t += eps;
-    }
-    QCB_HIP(c, hipGetLastError());
+        QCB_HIP(c, hipGetLastError());
+        return QCB_OK;
+    };
+    if ((rc = run_flow_graph(b, b->flow_graph[0], eps, n_steps, enqueue))) return rc;
     std::vector<double> h((size_t)b->n * nm * 2);
     QCB_HIP(c, hipMemcpyAsync(h.data(), b->d_meas, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     QCB_HIP(c, hipStreamSynchronize(c->stream));
+    std::vector<double> tv(nm);
+    double t = 0.0;   // flow time accumulated as the reference does
+    for (int s = 0; s < nm; ++s, t += eps) tv[s] = t;
     for (int i = 0; i < b->n; ++i) {
-        b->lats[i]->gstep = b->gstep;
+        b->lats[i]->gstep = (uint32_t)n_steps;
         if ((rc = upload_desc(b->lats[i]))) return rc;   // single-lattice calls see the flow state
         for (int s = 0; s < nm; ++s) {
             tQE[((size_t)i * nm + s) * 3] = tv[s];
@@ -433,39 +491,42 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
 // copied into its flow fields, n_steps RK3 steps run with one launch per stage
 // for all lattices, and the flowed fields are copied back into the links; no
 // measurement.  The flowed field stays the lattice's flow state, as after
-// qcb_flow_steps.
+// qcb_flow_steps.  Replayed from a captured graph like qcb_batch_flow.
 extern "C" int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps) {
     if (!b || n_steps < 0) return QCB_EINVAL;
     qcb_ctx* c = b->ctx;
     int rc;
     for (qcb_lattice* l : b->lats) {
         if ((rc = ensure_flow_buffers(l))) return rc;
-        l->flow_start = b->gstep;
+        l->flow_start = 0;
         l->flow_active = true;
     }
     if ((rc = batch_upload(b))) return rc;
-    for (qcb_lattice* l : b->lats) {
-        Launch one = lat_launch(l, kFamLink);
-        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
-        else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
-    }
-    Launch gl = batch_launch(b, kFamLink);
-    for (int s = 0; s < n_steps; ++s) {
-        for (int st = 1; st <= 3; ++st) {
-            if (b->prec == QCB_F64) launch_flow_stage<double>(gl, st, b->gstep, eps);
-            else launch_flow_stage<float>(gl, st, b->gstep, eps);
+    const bool f64 = b->prec == QCB_F64;
+    auto enqueue = [&]() -> int {
+        for (qcb_lattice* l : b->lats) {
+            Launch one = lat_launch(l, kFamLink);
+            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
+            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
         }
-        b->gstep++;
-    }
+        Launch gl = batch_launch(b, kFamLink);
+        for (int s = 0; s < n_steps; ++s)
+            for (int st = 1; st <= 3; ++st) {
+                if (f64) launch_flow_stage<double>(gl, st, (uint32_t)s, eps);
+                else launch_flow_stage<float>(gl, st, (uint32_t)s, eps);
+            }
+        for (qcb_lattice* l : b->lats) {
+            Launch one = lat_launch(l, kFamLink);
+            const void* cur = l->F[(uint32_t)n_steps & 1u];
+            if (f64) launch_copy_field<double>(one, (const double*)cur, (double*)l->U);
+            else launch_copy_field<float>(one, (const float*)cur, (float*)l->U);
+        }
+        QCB_HIP(c, hipGetLastError());
+        return QCB_OK;
+    };
+    if ((rc = run_flow_graph(b, b->flow_graph[1], eps, n_steps, enqueue))) return rc;
     for (qcb_lattice* l : b->lats) {
-        Launch one = lat_launch(l, kFamLink);
-        const void* cur = l->F[(b->gstep - l->flow_start) & 1u];
-        if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)cur, (double*)l->U);
-        else launch_copy_field<float>(one, (const float*)cur, (float*)l->U);
-    }
-    QCB_HIP(c, hipGetLastError());
-    for (qcb_lattice* l : b->lats) {
-        l->gstep = b->gstep;
+        l->gstep = (uint32_t)n_steps;
         if ((rc = upload_desc(l))) return rc;   // single-lattice calls see the flow state
     }
     return QCB_OK;

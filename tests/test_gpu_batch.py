@@ -120,6 +120,39 @@ def test_batch_flow_steps_equal_alone(ctx, prec):
     b.close()
 
 
+@pytest.mark.parametrize("prec", [qcb.F64, qcb.F32])
+def test_flow_graph_replays_equal_alone(ctx, prec):
+    """The batched flow is captured at its first call and replayed on later
+    calls with the same eps and step count: every replay, with heatbath sweeps
+    and synchronisation between, gives bitwise the single-lattice flow of the
+    links it starts from; a new eps or step count captures anew."""
+    b = qcb.Batch(ctx, DIMS, prec, BETAS, SEEDS)
+    b.hot()
+    for r, (eps, n) in enumerate([(0.02, 3), (0.02, 3), (0.02, 3), (0.01, 2), (0.02, 3)]):
+        b.hb_sweeps(2, 1, sweep_idx=r)
+        start = [b.lattice(i).get_links() for i in range(len(DIMS))]
+        flows = b.flow(eps, n)
+        for i, d in enumerate(DIMS):
+            g = qcb.Lattice(ctx, d, prec)
+            g.set_links(start[i])
+            t, fin = g.topo_charge_flowed(eps, n, 1, final=True)
+            assert np.array_equal(flows[i, :n], t), (r, i)
+            assert np.array_equal(flows[i, n], fin), (r, i)
+            g.close()
+    for r in range(3):
+        b.hb_sweeps(2, 1, sweep_idx=10 + r)
+        start = [b.lattice(i).get_links() for i in range(len(DIMS))]
+        b.flow_steps(0.02, 2)
+        for i, d in enumerate(DIMS):
+            g = qcb.Lattice(ctx, d, prec)
+            g.set_links(start[i])
+            g.flow_steps(0.02, 2)
+            assert np.array_equal(b.lattice(i).get_links(), g.get_links()), (r, i)
+            assert b.lattice(i).flow_qe() == g.flow_qe()
+            g.close()
+    b.close()
+
+
 def _config4():
     """64 lattices, 16 each of L = 8, 10, 12, 16 (L^4, periodic), beta ~ U[5.8, 6.2]
     from seed 3, one Philox key per lattice."""
