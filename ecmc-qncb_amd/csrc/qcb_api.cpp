@@ -221,15 +221,15 @@ extern "C" int qcb_lattice_set_links(qcb_lattice* lat, const double* aos) {
     return QCB_OK;
 }
 
-static int download_field(qcb_lattice* lat, const void* field, double* aos) {
+static int download_field(qcb_lattice* lat, const void* field, double* aos, int order) {
     qcb_ctx* ctx = lat->ctx;
     size_t n = (size_t)lat->V * 72;
     double* d = nullptr;
     DevTmp tmp(ctx);
     if (int rc = tmp.alloc(n * sizeof(double), &d)) return rc;
     Launch g = lat_launch(lat, kFamLink);
-    if (lat->prec == QCB_F64) launch_soa_to_aos<double>(g, (const double*)field, d);
-    else launch_soa_to_aos<float>(g, (const float*)field, d);
+    if (lat->prec == QCB_F64) launch_soa_to_aos<double>(g, (const double*)field, d, order);
+    else launch_soa_to_aos<float>(g, (const float*)field, d, order);
     QCB_HIP(ctx, hipGetLastError());
     QCB_HIP(ctx, hipMemcpyAsync(aos, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     QCB_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -238,7 +238,7 @@ static int download_field(qcb_lattice* lat, const void* field, double* aos) {
 
 extern "C" int qcb_lattice_get_links(qcb_lattice* lat, double* aos) {
     if (!lat || !aos) return QCB_EINVAL;
-    return download_field(lat, lat->U, aos);
+    return download_field(lat, lat->U, aos, kLinkOrder);
 }
 
 extern "C" int qcb_cold(qcb_lattice* lat) {
@@ -435,8 +435,8 @@ int flow_begin(qcb_lattice* lat) {
     rc = upload_desc(lat);
     if (rc) return rc;
     Launch g = lat_launch(lat, kFamLink);
-    if (lat->prec == QCB_F64) launch_copy_field<double>(g, (const double*)lat->U, (double*)lat->F[0]);
-    else launch_copy_field<float>(g, (const float*)lat->U, (float*)lat->F[0]);
+    if (lat->prec == QCB_F64) launch_copy_field<double>(g, (const double*)lat->U, (double*)lat->F[0], kLinkOrder, kFlowOrder);
+    else launch_copy_field<float>(g, (const float*)lat->U, (float*)lat->F[0], kLinkOrder, kFlowOrder);
     QCB_HIP(lat->ctx, hipGetLastError());
     return QCB_OK;
 }
@@ -450,8 +450,8 @@ extern "C" int qcb_flow_steps(qcb_lattice* lat, double eps, int n) {
     for (int s = 0; s < n; ++s)
         if ((rc = enqueue_flow_step(lat, eps))) return rc;
     Launch g = lat_launch(lat, kFamLink);
-    if (lat->prec == QCB_F64) launch_copy_field<double>(g, (const double*)flow_current(lat), (double*)lat->U);
-    else launch_copy_field<float>(g, (const float*)flow_current(lat), (float*)lat->U);
+    if (lat->prec == QCB_F64) launch_copy_field<double>(g, (const double*)flow_current(lat), (double*)lat->U, kFlowOrder, kLinkOrder);
+    else launch_copy_field<float>(g, (const float*)flow_current(lat), (float*)lat->U, kFlowOrder, kLinkOrder);
     QCB_HIP(lat->ctx, hipGetLastError());
     return QCB_OK;
 }
@@ -497,7 +497,7 @@ extern "C" int qcb_flow_plaquette(qcb_lattice* lat, double* p) {
 extern "C" int qcb_flow_get_links(qcb_lattice* lat, double* aos) {
     if (!lat || !aos) return QCB_EINVAL;
     if (!lat->flow_active) return qcb_set_error(lat->ctx, QCB_ESTATE, "qcb_flow_copy must come first");
-    return download_field(lat, flow_current(lat), aos);
+    return download_field(lat, flow_current(lat), aos, kFlowOrder);
 }
 
 extern "C" int qcb_topo_charge_flowed(qcb_lattice* lat, double eps, int n_steps_gf, int n_rk, double* tQE,

@@ -448,8 +448,8 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
         // copy links into the flow fields of every lattice: one launch per lattice
         for (qcb_lattice* l : b->lats) {
             Launch one = lat_launch(l, kFamLink);
-            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
-            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
+            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0], kLinkOrder, kFlowOrder);
+            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0], kLinkOrder, kFlowOrder);
         }
         for (int s = 0; s < nm; ++s) {
             if (f64) {
@@ -506,8 +506,8 @@ extern "C" int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps) {
     auto enqueue = [&]() -> int {
         for (qcb_lattice* l : b->lats) {
             Launch one = lat_launch(l, kFamLink);
-            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
-            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
+            if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0], kLinkOrder, kFlowOrder);
+            else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0], kLinkOrder, kFlowOrder);
         }
         Launch gl = batch_launch(b, kFamLink);
         for (int s = 0; s < n_steps; ++s)
@@ -518,8 +518,8 @@ extern "C" int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps) {
         for (qcb_lattice* l : b->lats) {
             Launch one = lat_launch(l, kFamLink);
             const void* cur = l->F[(uint32_t)n_steps & 1u];
-            if (f64) launch_copy_field<double>(one, (const double*)cur, (double*)l->U);
-            else launch_copy_field<float>(one, (const float*)cur, (float*)l->U);
+            if (f64) launch_copy_field<double>(one, (const double*)cur, (double*)l->U, kFlowOrder, kLinkOrder);
+            else launch_copy_field<float>(one, (const float*)cur, (float*)l->U, kFlowOrder, kLinkOrder);
         }
         QCB_HIP(c, hipGetLastError());
         return QCB_OK;

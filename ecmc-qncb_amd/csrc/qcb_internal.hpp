@@ -29,8 +29,10 @@ template <class T> void launch_aos_to_soa(const Launch& g, const double* aos, T*
 // ILDG staging of links [lo, lo + n) in file order (single-lattice launches)
 template <class T> void launch_soa_to_be(const Launch& g, const T* src, uint64_t* out, uint64_t lo, uint32_t n);
 template <class T> void launch_be_to_soa(const Launch& g, const uint64_t* in, T* dst, uint64_t lo, uint32_t n);
-template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos);
-template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst);
+// order: the storage order of the field (kLinkOrder for U, kFlowOrder for F[0], F[1])
+template <class T> void launch_soa_to_aos(const Launch& g, const T* src_field, double* aos, int order = kLinkOrder);
+template <class T> void launch_copy_field(const Launch& g, const T* src, T* dst, int from = kLinkOrder,
+                                          int to = kLinkOrder);
 template <class T> void launch_flow_init(const Launch& g, Gate gate = Gate{});   // F[0] <- U
 // which = 0: the links; 1: each lattice's current flow field at flow step gstep
 template <class T> void launch_plaquette(const Launch& g, double* partial, int which = 0, uint32_t gstep = 0);

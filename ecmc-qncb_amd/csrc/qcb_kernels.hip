@@ -131,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_aos_to_soa(const LatDev<T>* __restri
     st_link(dst, L.S, mu, p, i, m);
 }
 
-template <class T>
+template <class T, int LAY>
 __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, const T* __restrict__ src,
                                                        double* __restrict__ aos) {
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_soa_to_aos(const LatDev<T>* __restri
     Coords c = cb_coords(L, i, p);
     size_t lex = (size_t)c.x + (size_t)L.L[0] * ((size_t)c.y + (size_t)L.L[1] * ((size_t)c.z + (size_t)L.L[2] * (size_t)c.t));
     double* d = aos + (lex * 4 + (size_t)mu) * 18;
-    M3<T> m = ld_link(src, L.S, mu, p, i);
+    M3<T> m = ld_link<T, LAY>(src, L.S, mu, p, i);
 #pragma unroll
     for (int k = 0; k < 18; ++k) d[k] = (double)m.v[k];
 }
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_be_to_soa(const LatDev<T>* __restric
     st_link(dst, L.S, mu, p, lex >> 1, m);
 }
 
-template <class T>
+template <class T, int FROM, int TO>
 __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, const T* __restrict__ src,
                                                        T* __restrict__ dst) {
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_field(const LatDev<T>* __restri
     const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
     const uint32_t hs = threadIdx.x / kLinkSites;
     if (i >= L.Vh) return;
-    st_link(dst, L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(src, L.S, (int)(hs >> 1), (int)(hs & 1), i));
+    st_link<T, TO>(dst, L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link<T, FROM>(src, L.S, (int)(hs >> 1), (int)(hs & 1), i));
 }
 
 // Flow start for every lattice of a batch: F[0] <- U (GradientFlow::copy).
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_flow_init(const LatDev<T>* __restric
     const uint32_t i = bc.local * kLinkSites + (threadIdx.x & (kLinkSites - 1));
     const uint32_t hs = threadIdx.x / kLinkSites;
     if (i >= L.Vh) return;
-    st_link(L.F[0], L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i));
+    st_link<T, kFlowOrder>(L.F[0], L.S, (int)(hs >> 1), (int)(hs & 1), i, ld_link(L.U, L.S, (int)(hs >> 1), (int)(hs & 1), i));
 }
 
 // U'(x) = U(x - s): each thread writes one destination link.
@@ -261,9 +261,9 @@ __global__ __launch_bounds__(kBlock) void k_reunit(const LatDev<T>* __restrict__
 // Per-site sum over mu<nu of Re Tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag)/3
 // (reference src/observables/observables_mpi.cpp:8-30), FP64 accumulation,
 // of the links (which = 0) or of the lattice's current flow field (which = 1).
-template <class T>
+template <class T, int LAY>
 __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                      int n_lat, uint32_t nblocks, int which, uint32_t gstep,
+                                                      int n_lat, uint32_t nblocks, uint32_t gstep,
                                                       double* __restrict__ partial) {
     __shared__ double lds[kBlock / 64];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
@@ -274,16 +274,16 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
     if (qs < L.Vh) {
         Coords c = tiled_coords(L, qs, p);
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-        const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
+        const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
 #pragma unroll
         for (int mu = 0; mu < 3; ++mu) {
-            M3<T> umu = ld_link(f, L.S, mu, p, i);
+            M3<T> umu = ld_link<T, LAY>(f, L.S, mu, p, i);
             const uint32_t xmu = nb_cb(L, c, mu, +1);
 #pragma unroll
             for (int nu = mu + 1; nu < 4; ++nu) {
                 const uint32_t xnu = nb_cb(L, c, nu, +1);
-                M3<T> a = mul_nn(umu, ld_link(f, L.S, nu, p ^ 1, xmu));
-                M3<T> b = mul_nn(ld_link(f, L.S, nu, p, i), ld_link(f, L.S, mu, p ^ 1, xnu));
+                M3<T> a = mul_nn(umu, ld_link<T, LAY>(f, L.S, nu, p ^ 1, xmu));
+                M3<T> b = mul_nn(ld_link<T, LAY>(f, L.S, nu, p, i), ld_link<T, LAY>(f, L.S, mu, p ^ 1, xnu));
                 sum += (double)retrace_nd(a, b) / 3.0;
             }
         }
@@ -299,21 +299,21 @@ __global__ __launch_bounds__(kBlock) void k_plaquette(const LatDev<T>* __restric
 // a b (or a b^dag) with the SU(3) factor a read from memory: an FP32 link
 // gives its two stored rows, rows 0 and 1 of the product follow and row 2 is
 // rebuilt; FP64 forms the full product
-template <class T, bool DAG>
+template <class T, bool DAG, int LAY>
 __device__ __forceinline__ M3<T> mul_ld(const T* __restrict__ f, uint32_t S, int mu, int p, uint32_t i,
                                         const M3<T>& b) {
     if constexpr (LinkSlots<T>::n == 6) {
-        const M3<T> a = ld_link_rows01(f, S, mu, p, i);
+        const M3<T> a = ld_link_rows01<LAY>(f, S, mu, p, i);
         M3<T> t = DAG ? mul_nd<T, 2>(a, b) : mul_nn<T, 2>(a, b);
         su3_third_row(t);
         return t;
     } else {
-        const M3<T> a = ld_link(f, S, mu, p, i);
+        const M3<T> a = ld_link<T, LAY>(f, S, mu, p, i);
         return DAG ? mul_nd(a, b) : mul_nn(a, b);
     }
 }
 
-template <class T>
+template <class T, int LAY>
 __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i,
                                           const int MU, const int NU) {
     const uint32_t S = L.S;
@@ -324,27 +324,27 @@ __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restric
     const uint32_t xpmumnu = nb_cb(L, c, MU, +1, NU, -1);
     M3<T> C;
     {   // U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
-        M3<T> a = mul_ld<T, false>(f, S, MU, p, i, ld_link(f, S, NU, q, xpmu));
-        M3<T> b = mul_ld<T, false>(f, S, NU, p, i, ld_link(f, S, MU, q, xpnu));
+        M3<T> a = mul_ld<T, false, LAY>(f, S, MU, p, i, ld_link<T, LAY>(f, S, NU, q, xpmu));
+        M3<T> b = mul_ld<T, false, LAY>(f, S, NU, p, i, ld_link<T, LAY>(f, S, MU, q, xpnu));
         C = mul_nd(a, b);
     }
     __builtin_amdgcn_sched_barrier(0);   // one leaf's loads at a time: keeps the registers down
     {   // U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
-        M3<T> a = mul_ld<T, true>(f, S, NU, p, i, ld_link(f, S, MU, p, xmmupnu));
-        M3<T> b = mul_dn(ld_link(f, S, MU, q, xmmu), ld_link(f, S, NU, q, xmmu));
+        M3<T> a = mul_ld<T, true, LAY>(f, S, NU, p, i, ld_link<T, LAY>(f, S, MU, p, xmmupnu));
+        M3<T> b = mul_dn(ld_link<T, LAY>(f, S, MU, q, xmmu), ld_link<T, LAY>(f, S, NU, q, xmmu));
         acc_nd(C, a, b);
     }
     __builtin_amdgcn_sched_barrier(0);
     {   // U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
-        M3<T> a = mul_ld<T, false>(f, S, NU, p, xmmumnu, ld_link(f, S, MU, q, xmmu));
-        M3<T> b = mul_ld<T, false>(f, S, MU, p, xmmumnu, ld_link(f, S, NU, q, xmnu));
+        M3<T> a = mul_ld<T, false, LAY>(f, S, NU, p, xmmumnu, ld_link<T, LAY>(f, S, MU, q, xmmu));
+        M3<T> b = mul_ld<T, false, LAY>(f, S, MU, p, xmmumnu, ld_link<T, LAY>(f, S, NU, q, xmnu));
         acc_dn(C, a, b);
     }
     __builtin_amdgcn_sched_barrier(0);
     {   // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
-        M3<T> a = mul_ld<T, false>(f, S, MU, q, xmnu, ld_link(f, S, NU, p, xpmumnu));
-        M3<T> b = mul_dn(ld_link(f, S, NU, q, xmnu), a);
-        acc_nd(C, b, ld_link(f, S, MU, p, i));
+        M3<T> a = mul_ld<T, false, LAY>(f, S, MU, q, xmnu, ld_link<T, LAY>(f, S, NU, p, xpmumnu));
+        M3<T> b = mul_dn(ld_link<T, LAY>(f, S, NU, q, xmnu), a);
+        acc_nd(C, b, ld_link<T, LAY>(f, S, MU, p, i));
         // C now holds all four leaves (acc_nd adds b * U_mu(x)^dag)
     }
     M3<T> F;
@@ -368,9 +368,9 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 
 // q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
 // (reference src/observables/observables_mpi.cpp:88-107).
-template <class T>
+template <class T, int LAY>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::clover))) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
-                                                   int n_lat, uint32_t nblocks, int which, uint32_t gstep,
+                                                   int n_lat, uint32_t nblocks, uint32_t gstep,
                                                    double* __restrict__ pq, double* __restrict__ pe) {
     __shared__ double lds[kBlock / 64];
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     if (qs < L.Vh) {
         Coords c = tiled_coords(L, qs, p);
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-        const T* f = which == 0 ? L.U : L.F[(gstep - L.flow_start) & 1u];
+        const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
         double tq = 0.0;
         // the three plane pairs (01,23), (02,13), (03,12) one at a time: a
         // loop the compiler keeps rolled bounds the registers to one pair
@@ -389,8 +389,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         for (int pr = 0; pr < 3; ++pr) {
             const int nu = pr + 1;                            // (0, nu) and its complement
             const int m2 = pr == 2 ? 1 : (pr == 1 ? 1 : 2), n2 = pr == 0 ? 3 : (pr == 1 ? 3 : 2);
-            M3<T> a = clover_F<T>(L, f, c, p, i, 0, nu);
-            M3<T> b = clover_F<T>(L, f, c, p, i, m2, n2);
+            M3<T> a = clover_F<T, LAY>(L, f, c, p, i, 0, nu);
+            M3<T> b = clover_F<T, LAY>(L, f, c, p, i, m2, n2);
             const double tr = (double)retrace_nn(a, b);
             tq += pr == 1 ? -tr : tr;
             e += sqnorm(a) + sqnorm(b);
@@ -703,9 +703,8 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 //  stage 3:                              W' = exp(eps (3/4 Z - Zs)) W
 // Z and Zs are carried as their 9 independent reals (A9).
 template <class T, int STAGE>
-__device__ __forceinline__ void flow_tail(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
+__device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T* __restrict__ wout,
                                           T* __restrict__ zs, int mu, int p, uint32_t i, T eps, const M3<T>& Sg) {
-    M3<T> W = ld_link(win, L.S, mu, p, i);
     A9<T> Z = force_ta(W, Sg);
     A9<T> X;
     if (STAGE == 1) {
@@ -723,7 +722,7 @@ __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const T* __restric
     M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
     M3<T> Wn = mul_nn(E, W);
     su3_store_fix(Wn);
-    st_link(wout, L.S, mu, p, i, Wn);
+    st_link<T, kFlowOrder>(wout, L.S, mu, p, i, Wn);
 }
 
 // The block's 256 threads are 32 sites x 8 (mu, parity) classes, so the four
@@ -732,9 +731,15 @@ __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const T* __restric
 // staples through x,
 //   Sigma_a += S U_b(x)^dag,   Sigma_b += S^dag U_a(x)^dag,
 // so one of the two threads forms it (owners spread 2/1/2/1 over the four
-// wavefronts) and hands it to the other through LDS; the lower staples
-// U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b) are not shared.  Phase 1: owned S into
-// LDS, then the three lower staples; barrier; phase 2: the three upper
+// wavefronts) and hands it to the other through LDS.  The upper staples of
+// the four links of x also share the links U_nu(x) themselves; these kernels
+// are bound by their loads more than by their arithmetic (an FP32 stage with
+// all nine complex entries stored, which skips every row-2 rebuild, runs 37%
+// slower), so in FP32 each thread publishes its own link in LDS and the other
+// three read it there.  The lower staples U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b)
+// are not shared (publishing their common U_b(x-b) as well costs a block of
+// residency per CU and gained nothing).  Phase 1: own link (FP32) and owned S
+// into LDS, then the three lower staples; barrier; phase 2: the three upper
 // staples and the update.
 __device__ __forceinline__ constexpr int plane_idx(int a, int b) { return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5); }
 __device__ __forceinline__ constexpr int plane_owner(int pl) {
@@ -745,66 +750,125 @@ __device__ __forceinline__ constexpr uint32_t plane_slot(int pl, int p, int k, u
     return ((uint32_t)(pl * 2 + p) * 18u + (uint32_t)k) * kLinkSites + s;
 }
 
-template <class T, int MU>
-__device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
-                                            uint32_t s, T* lds, M3<T>& Sg) {
-    m3_zero(Sg);
-    const uint32_t S = L.S;
-    const int q = p ^ 1;
+// LDS of one flow-stage block: each plane's upper-staple product S per parity
+// and site; in FP32 also the stored rows of every site's four links U_mu(x).
+template <class T>
+struct FlowLds {
+    T s[6 * 2 * 18 * kLinkSites];
+};
+template <>
+struct FlowLds<float> {
+    typedef Vec2<float>::type C2;
+    float s[6 * 2 * 18 * kLinkSites];
+    C2 w[8 * 6 * kLinkSites];   // U_mu(x): [(hs * 6 + entry) * 32 + site], hs = 2 mu + parity
+};
+
+__device__ __forceinline__ void lds_put_rows01(Vec2<float>::type* a, int hs, uint32_t s, const M3<float>& m) {
 #pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
-        if (nu == MU) continue;
-        const int a = MU < nu ? MU : nu, b = MU < nu ? nu : MU;
-        const int pl = plane_idx(a, b);
-        if (plane_owner(pl) == MU) {
-            const uint32_t xa = nb_cb(L, c, a, +1), xb = nb_cb(L, c, b, +1);
-            M3<T> t;
-            if constexpr (LinkSlots<T>::n == 6) {
-                M3<T> u = ld_link_rows01(win, S, b, q, xa);
-                t = mul_nd<T, 2>(u, ld_link(win, S, a, q, xb));
-                su3_third_row(t);
-            } else {
-                t = mul_nd(ld_link(win, S, b, q, xa), ld_link(win, S, a, q, xb));
-            }
+    for (int k = 0; k < 6; ++k) a[((uint32_t)hs * 6u + (uint32_t)k) * kLinkSites + s] = {m.v[2 * k], m.v[2 * k + 1]};
+}
+__device__ __forceinline__ M3<float> lds_get_link(const Vec2<float>::type* a, int hs, uint32_t s) {
+    M3<float> m;
 #pragma unroll
-            for (int k = 0; k < 18; ++k) lds[plane_slot(pl, p, k, s)] = t.v[k];
-        }
-        const uint32_t xmnu = nb_cb(L, c, nu, -1);
-        const uint32_t xmumnu = nb_cb(L, c, MU, +1, nu, -1);
+    for (int k = 0; k < 6; ++k) {
+        const Vec2<float>::type c = a[((uint32_t)hs * 6u + (uint32_t)k) * kLinkSites + s];
+        m.v[2 * k] = c.x;
+        m.v[2 * k + 1] = c.y;
+    }
+    su3_third_row(m);
+    return m;
+}
+
+// The owned plane's upper-staple start S = U_b(x+a) U_a(x+b)^dag into LDS.
+template <class T, int MU, int NU>
+__device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
+                                               uint32_t s, T* lds) {
+    constexpr int a = MU < NU ? MU : NU, b = MU < NU ? NU : MU;
+    constexpr int pl = plane_idx(a, b);
+    if constexpr (plane_owner(pl) == MU) {
+        const int q = p ^ 1;
+        const uint32_t xa = nb_cb(L, c, a, +1), xb = nb_cb(L, c, b, +1);
         M3<T> t;
         if constexpr (LinkSlots<T>::n == 6) {
-            M3<T> u = ld_link_rows01(win, S, MU, q, xmnu);   // U_mu(x-nu)
-            t = mul_nn<T, 2>(u, ld_link(win, S, nu, p, xmumnu));
+            M3<T> u = ld_link_rows01<kFlowOrder>(win, L.S, b, q, xa);
+            t = mul_nd<T, 2>(u, ld_link<T, kFlowOrder>(win, L.S, a, q, xb));
             su3_third_row(t);
         } else {
-            t = mul_nn(ld_link(win, S, MU, q, xmnu), ld_link(win, S, nu, p, xmumnu));
+            t = mul_nd(ld_link<T, kFlowOrder>(win, L.S, b, q, xa), ld_link<T, kFlowOrder>(win, L.S, a, q, xb));
         }
-        acc_dn(Sg, t, ld_link(win, S, nu, q, xmnu));           // .. U_nu(x-nu)
+#pragma unroll
+        for (int k = 0; k < 18; ++k) lds[plane_slot(pl, p, k, s)] = t.v[k];
     }
+}
+
+// Sigma += the lower staple U_nu(x+mu-nu)^dag U_mu(x-nu)^dag U_nu(x-nu), with
+// U_nu(x-nu) given.
+template <class T, int MU, int NU>
+__device__ __forceinline__ void flow_lower(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
+                                           const M3<T>& down, M3<T>& Sg) {
+    const int q = p ^ 1;
+    const uint32_t xmnu = nb_cb(L, c, NU, -1);
+    const uint32_t xmumnu = nb_cb(L, c, MU, +1, NU, -1);
+    M3<T> t;
+    if constexpr (LinkSlots<T>::n == 6) {
+        M3<T> u = ld_link_rows01<kFlowOrder>(win, L.S, MU, q, xmnu);   // U_mu(x-nu)
+        t = mul_nn<T, 2>(u, ld_link<T, kFlowOrder>(win, L.S, NU, p, xmumnu));
+        su3_third_row(t);
+    } else {
+        t = mul_nn(ld_link<T, kFlowOrder>(win, L.S, MU, q, xmnu), ld_link<T, kFlowOrder>(win, L.S, NU, p, xmumnu));
+    }
+    acc_dn(Sg, t, down);
+}
+
+// Sigma += the upper staple through x from its plane's S in LDS and U_nu(x).
+template <class T, int MU, int NU>
+__device__ __forceinline__ void flow_upper(const T* lds, int p, uint32_t s, const M3<T>& unu, M3<T>& Sg) {
+    constexpr int pl = plane_idx(MU < NU ? MU : NU, MU < NU ? NU : MU);
+    M3<T> t;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) t.v[k] = lds[plane_slot(pl, p, k, s)];
+    if constexpr (MU < NU) acc_nd(Sg, t, unu);
+    else acc_dd(Sg, t, unu);
+}
+
+template <class T, int MU>
+__device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
+                                            uint32_t s, uint32_t i, FlowLds<T>& sh, M3<T>& Sg) {
+    if constexpr (LinkSlots<T>::n == 6) lds_put_rows01(sh.w, MU * 2 + p, s, ld_link_rows01<kFlowOrder>(win, L.S, MU, p, i));
+    m3_zero(Sg);
+    const int q = p ^ 1;
+    constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
+    flow_put_plane<T, MU, N0>(L, win, c, p, s, sh.s);
+    flow_lower<T, MU, N0>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N0, q, nb_cb(L, c, N0, -1)), Sg);
+    flow_put_plane<T, MU, N1>(L, win, c, p, s, sh.s);
+    flow_lower<T, MU, N1>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N1, q, nb_cb(L, c, N1, -1)), Sg);
+    flow_put_plane<T, MU, N2>(L, win, c, p, s, sh.s);
+    flow_lower<T, MU, N2>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N2, q, nb_cb(L, c, N2, -1)), Sg);
+}
+
+// U_nu(x) for the update of a link of x: from LDS in FP32, else from the field.
+template <class T>
+__device__ __forceinline__ M3<T> site_link(const LatDev<T>& L, const T* __restrict__ win, const FlowLds<T>& sh,
+                                           int nu, int p, uint32_t i, uint32_t s) {
+    if constexpr (LinkSlots<T>::n == 6) return lds_get_link(sh.w, nu * 2 + p, s);
+    else return ld_link<T, kFlowOrder>(win, L.S, nu, p, i);
 }
 
 template <class T, int MU, int STAGE>
 __device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
-                                            T* __restrict__ zs, int p, uint32_t i, uint32_t s, const T* lds,
+                                            T* __restrict__ zs, int p, uint32_t i, uint32_t s, const FlowLds<T>& sh,
                                             M3<T>& Sg, T eps) {
-#pragma unroll
-    for (int nu = 0; nu < 4; ++nu) {
-        if (nu == MU) continue;
-        const int pl = plane_idx(MU < nu ? MU : nu, MU < nu ? nu : MU);
-        M3<T> t;
-#pragma unroll
-        for (int k = 0; k < 18; ++k) t.v[k] = lds[plane_slot(pl, p, k, s)];
-        const M3<T> d = ld_link(win, L.S, nu, p, i);   // U_nu(x)
-        if (MU < nu) acc_nd(Sg, t, d);
-        else acc_dd(Sg, t, d);
-    }
-    flow_tail<T, STAGE>(L, win, wout, zs, MU, p, i, eps, Sg);
+    constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
+    flow_upper<T, MU, N0>(sh.s, p, s, site_link(L, win, sh, N0, p, i, s), Sg);
+    flow_upper<T, MU, N1>(sh.s, p, s, site_link(L, win, sh, N1, p, i, s), Sg);
+    flow_upper<T, MU, N2>(sh.s, p, s, site_link(L, win, sh, N2, p, i, s), Sg);
+    flow_tail<T, STAGE>(L, site_link(L, win, sh, MU, p, i, s), wout, zs, MU, p, i, eps, Sg);
 }
 
 template <class T, int STAGE>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::flow))) void k_flow_stage(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                        int n_lat, uint32_t nblocks, uint32_t gstep, T eps) {
-    __shared__ T lds[6 * 2 * 18 * kLinkSites];
+    __shared__ FlowLds<T> sh;
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
     if (gstep - L.flow_start >= L.flow_limit) return;   // the whole block: its lattice has finished flowing
@@ -824,19 +888,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     M3<T> Sg;
     if (live) {
         switch (mu) {
-            case 0: flow_phase1<T, 0>(L, win, c, p, s, lds, Sg); break;
-            case 1: flow_phase1<T, 1>(L, win, c, p, s, lds, Sg); break;
-            case 2: flow_phase1<T, 2>(L, win, c, p, s, lds, Sg); break;
-            default: flow_phase1<T, 3>(L, win, c, p, s, lds, Sg); break;
+            case 0: flow_phase1<T, 0>(L, win, c, p, s, i, sh, Sg); break;
+            case 1: flow_phase1<T, 1>(L, win, c, p, s, i, sh, Sg); break;
+            case 2: flow_phase1<T, 2>(L, win, c, p, s, i, sh, Sg); break;
+            default: flow_phase1<T, 3>(L, win, c, p, s, i, sh, Sg); break;
         }
     }
     __syncthreads();
     if (live) {
         switch (mu) {
-            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
-            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
-            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
-            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, p, i, s, lds, Sg, eps); break;
+            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
+            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
+            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
+            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
         }
     }
 }
@@ -917,8 +981,11 @@ void launch_aos_to_soa(const Launch& g, const double* aos, T* dst) {
     hipLaunchKernelGGL(k_aos_to_soa<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), aos, dst);
 }
 template <class T>
-void launch_soa_to_aos(const Launch& g, const T* src, double* aos) {
-    hipLaunchKernelGGL(k_soa_to_aos<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, aos);
+void launch_soa_to_aos(const Launch& g, const T* src, double* aos, int order) {
+    if (order == kFlowOrder)
+        hipLaunchKernelGGL((k_soa_to_aos<T, kFlowOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, aos);
+    else
+        hipLaunchKernelGGL((k_soa_to_aos<T, kLinkOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, aos);
 }
 template <class T>
 void launch_soa_to_be(const Launch& g, const T* src, uint64_t* out, uint64_t lo, uint32_t n) {
@@ -931,8 +998,16 @@ void launch_be_to_soa(const Launch& g, const uint64_t* in, T* dst, uint64_t lo, 
                        static_cast<const LatDev<T>*>(g.lats), in, dst, lo, n);
 }
 template <class T>
-void launch_copy_field(const Launch& g, const T* src, T* dst) {
-    hipLaunchKernelGGL(k_copy_field<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), src, dst);
+void launch_copy_field(const Launch& g, const T* src, T* dst, int from, int to) {
+    const dim3 gr(g.nblocks), bl(kBlock);
+    if (from == kLinkOrder && to == kFlowOrder)
+        hipLaunchKernelGGL((k_copy_field<T, kLinkOrder, kFlowOrder>), gr, bl, 0, g.stream, QCB_ARGS(g), src, dst);
+    else if (from == kFlowOrder && to == kLinkOrder)
+        hipLaunchKernelGGL((k_copy_field<T, kFlowOrder, kLinkOrder>), gr, bl, 0, g.stream, QCB_ARGS(g), src, dst);
+    else if (from == kFlowOrder)
+        hipLaunchKernelGGL((k_copy_field<T, kFlowOrder, kFlowOrder>), gr, bl, 0, g.stream, QCB_ARGS(g), src, dst);
+    else
+        hipLaunchKernelGGL((k_copy_field<T, kLinkOrder, kLinkOrder>), gr, bl, 0, g.stream, QCB_ARGS(g), src, dst);
 }
 template <class T>
 void launch_flow_init(const Launch& g, Gate gate) {
@@ -940,12 +1015,21 @@ void launch_flow_init(const Launch& g, Gate gate) {
 }
 template <class T>
 void launch_plaquette(const Launch& g, double* partial, int which, uint32_t gstep) {
-    hipLaunchKernelGGL(k_plaquette<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), which, gstep,
-                       partial);
+    if (which)
+        hipLaunchKernelGGL((k_plaquette<T, kFlowOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g),
+                           gstep, partial);
+    else
+        hipLaunchKernelGGL((k_plaquette<T, kLinkOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g),
+                           gstep, partial);
 }
 template <class T>
 void launch_clover(const Launch& g, int which, uint32_t gstep, double* pq, double* pe) {
-    hipLaunchKernelGGL(k_clover<T>, dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), which, gstep, pq, pe);
+    if (which)
+        hipLaunchKernelGGL((k_clover<T, kFlowOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gstep,
+                           pq, pe);
+    else
+        hipLaunchKernelGGL((k_clover<T, kLinkOrder>), dim3(g.nblocks), dim3(kBlock), 0, g.stream, QCB_ARGS(g), gstep,
+                           pq, pe);
 }
 template <class T>
 void launch_staples(const Launch& g, double* out) {
@@ -1019,8 +1103,8 @@ void launch_finalize_qe(const Launch& g, const double* pq, const double* pe, dou
     template void launch_cold<T>(const Launch&);                                                              \
     template void launch_hot<T>(const Launch&, Gate);                                                         \
     template void launch_aos_to_soa<T>(const Launch&, const double*, T*);                                     \
-    template void launch_soa_to_aos<T>(const Launch&, const T*, double*);                                     \
-    template void launch_copy_field<T>(const Launch&, const T*, T*);                                          \
+    template void launch_soa_to_aos<T>(const Launch&, const T*, double*, int);                                \
+    template void launch_copy_field<T>(const Launch&, const T*, T*, int, int);                                \
     template void launch_soa_to_be<T>(const Launch&, const T*, uint64_t*, uint64_t, uint32_t);                 \
     template void launch_be_to_soa<T>(const Launch&, const uint64_t*, T*, uint64_t, uint32_t);                 \
     template void launch_flow_init<T>(const Launch&, Gate);                                                   \
