@@ -107,6 +107,17 @@ inline uint32_t fam_blocks(uint32_t Vh, int fam) {
     return (Vh + per - 1) / per;
 }
 
+// The multiplier of FastDiv for divisor d >= 1.
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    FastDiv f;
+    f.m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    f.s1 = l ? 1u : 0u;
+    f.s2 = l ? l - 1 : 0u;
+    return f;
+}
+
 template <class T>
 inline LatDev<T> make_desc(const qcb_lattice* l) {
     LatDev<T> d{};
@@ -139,6 +150,12 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
         while ((1 << b) < d.NT[k]) ++b;
         d.tbits[k] = b;
     }
+    d.dv_tvol = make_fastdiv(d.tvol);
+    d.dv_lxh = make_fastdiv((uint32_t)d.Lxh);
+    d.dv_t1 = make_fastdiv((uint32_t)d.TL[1]);
+    d.dv_t2 = make_fastdiv((uint32_t)d.TL[2]);
+    d.dv_nt1 = make_fastdiv((uint32_t)d.NT[1]);
+    d.dv_nt2 = make_fastdiv((uint32_t)d.NT[2]);
     return d;
 }
 

@@ -30,6 +30,20 @@ namespace qcb {
 constexpr int kBlock = 256;          // threads per block of every lattice kernel
 constexpr int kNumXcd = 8;           // MI355X: 8 XCDs, blocks dispatched round-robin
 
+// n / d for any 32-bit n by a multiplier fixed per divisor (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", PLDI
+// 1994, sec. 4): with l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+// t = mulhi(m, n) and n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0).
+// One quarter-rate multiply instead of a reciprocal-and-correct sequence.
+struct FastDiv {
+    uint32_t m, s1, s2;
+};
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+    const uint32_t t = __umulhi(n, f.m);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
 // Per-lattice descriptor, resident in device memory; one entry per lattice of
 // a batch (a single lattice is a batch of one).
 template <class T>
@@ -53,6 +67,7 @@ struct LatDev {
     unsigned long long* kp_exh;  // device count of draws that ran out of attempts (skipped updates)
     uint32_t flow_limit;  // RK3 steps this lattice takes after flow_start (later stage launches skip it)
     double* qe_out;       // if set, a measurement at flow step s writes (Q, E/V) to qe_out[2s], qe_out[2s+1]
+    FastDiv dv_tvol, dv_lxh, dv_t1, dv_t2, dv_nt1, dv_nt2;   // by tvol, Lxh, TL[1], TL[2], NT[1], NT[2]
 };
 
 // A per-lattice cadence for kernels launched over a set of lattices that are
@@ -92,16 +107,18 @@ __device__ __forceinline__ uint32_t cb_index(const LatDev<T>& L, int x, int y, i
 }
 
 // Coordinates of the q-th site of parity p in the tiled work order.
+// The divisions use the descriptor's multipliers and the remainders 24-bit
+// products (tile counts and tile volumes stay below 2^24).
 template <class T>
 __device__ __forceinline__ Coords tiled_coords(const LatDev<T>& L, uint32_t q, int p) {
-    uint32_t tau = q / L.tvol;
-    uint32_t r = q - tau * L.tvol;
-    uint32_t r1 = r / (uint32_t)L.Lxh;
-    int xh = (int)(r - r1 * (uint32_t)L.Lxh);
-    uint32_t r2 = r1 / (uint32_t)L.TL[1];
-    int yl = (int)(r1 - r2 * (uint32_t)L.TL[1]);
-    uint32_t r3 = r2 / (uint32_t)L.TL[2];
-    int zl = (int)(r2 - r3 * (uint32_t)L.TL[2]);
+    uint32_t tau = fdiv(q, L.dv_tvol);
+    uint32_t r = q - __umul24(tau, L.tvol);
+    uint32_t r1 = fdiv(r, L.dv_lxh);
+    int xh = (int)(r - __umul24(r1, (uint32_t)L.Lxh));
+    uint32_t r2 = fdiv(r1, L.dv_t1);
+    int yl = (int)(r1 - __umul24(r2, (uint32_t)L.TL[1]));
+    uint32_t r3 = fdiv(r2, L.dv_t2);
+    int zl = (int)(r2 - __umul24(r3, (uint32_t)L.TL[2]));
     int tl = (int)r3;
     int ty, tz, tt;
     if (L.morton) {
@@ -115,16 +132,16 @@ __device__ __forceinline__ Coords tiled_coords(const LatDev<T>& L, uint32_t q, i
             if (b < L.tbits[3]) tt |= (int)((tau >> pos++) & 1u) << b;
         }
     } else {
-        uint32_t u1 = tau / (uint32_t)L.NT[1];
-        ty = (int)(tau - u1 * (uint32_t)L.NT[1]);
-        uint32_t u2 = u1 / (uint32_t)L.NT[2];
-        tz = (int)(u1 - u2 * (uint32_t)L.NT[2]);
+        uint32_t u1 = fdiv(tau, L.dv_nt1);
+        ty = (int)(tau - __umul24(u1, (uint32_t)L.NT[1]));
+        uint32_t u2 = fdiv(u1, L.dv_nt2);
+        tz = (int)(u1 - __umul24(u2, (uint32_t)L.NT[2]));
         tt = (int)u2;
     }
     Coords c;
-    c.y = ty * L.TL[1] + yl;
-    c.z = tz * L.TL[2] + zl;
-    c.t = tt * L.TL[3] + tl;
+    c.y = (int)__umul24((uint32_t)ty, (uint32_t)L.TL[1]) + yl;
+    c.z = (int)__umul24((uint32_t)tz, (uint32_t)L.TL[2]) + zl;
+    c.t = (int)__umul24((uint32_t)tt, (uint32_t)L.TL[3]) + tl;
     c.x = 2 * xh + ((c.y + c.z + c.t + p) & 1);
     return c;
 }
