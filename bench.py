@@ -57,6 +57,20 @@ def time_hb(qcb, ctx, L, prec, steps, warmup, n_hits=2, beta=6.0, seed=11):
                 achieved_gbs=achieved, plaquette=plaq)
 
 
+def warm_device(qcb, ctx, L, secs=0.5):
+    """Heatbath sweeps on an FP32 L^4 lattice for secs of wall time, so that
+    the first timed leg starts at the clocks the GPU holds under load (a cold
+    device runs its first ~0.1-0.3 s of work ~10% slower)."""
+    lat = qcb.Lattice(ctx, (L,) * 4, qcb.F32)
+    lat.hot(1)
+    t0, k = time.perf_counter(), 0
+    while time.perf_counter() - t0 < secs:
+        lat.hb_sweeps(6.0, 2, 10, seed=1, sweep_idx=k)
+        ctx.sync()
+        k += 10
+    lat.close()
+
+
 def time_wf(qcb, ctx, L, prec, steps, warmup, eps=0.02, seed=11):
     lat = qcb.Lattice(ctx, (L,) * 4, prec)
     lat.hot(seed)
@@ -401,6 +415,8 @@ def main():
     ap.add_argument("--precision-report", action="store_true",
                     help="run the FP32-vs-FP64 comparison (32^4, 500 sweeps, 150 flow steps) and print it")
     ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the flow-service leg and print its JSON (for a kernel trace of that leg)")
     ap.add_argument("--ecmc-report", action="store_true",
                     help="compare the event chains' equilibrium plaquette with the heatbath's (32^4, FP64) and print it")
     args = ap.parse_args()
@@ -420,12 +436,17 @@ def main():
         print(json.dumps(ecmc_report(qcb, ctx, L)))
         ctx.close()
         return
+    if args.serve_only:
+        print(json.dumps(serve(qcb, ctx, args.serve_requests)))
+        ctx.close()
+        return
     # The other legs run first and the headline last: the GPU then starts the
     # headline's own warmup sweeps already at its steady clocks under load
     # (on a cold device the first ~0.1 s of sweeps run ~10% slower).
     comp = {}
+    warm_device(qcb, ctx, L)
     log(f"[bench] WF FP32 {L}^4: {args.flow_steps} RK3 steps")
-    wf32 = time_wf(qcb, ctx, L, qcb.F32, args.flow_steps, 2)
+    wf32 = time_wf(qcb, ctx, L, qcb.F32, args.flow_steps, args.flow_steps)
     log(f"[bench] WF FP32 {wf32['site_steps_per_s']:.3e} site-steps/s, {wf32['achieved_gbs']:.0f} GB/s")
     comp["wf_fp32"] = wf32
     cl = {"flops_per_site": CLOVER_FLOPS_PER_SITE, "valu_peak_tflops": VALU_PEAK_TFLOPS,
@@ -435,9 +456,9 @@ def main():
     log(f"[bench] clover Q/E {L}^4: FP32 {cl['fp32']['ms']:.3f} ms ({cl['fp32']['tflops']:.1f} TFLOP/s)")
     comp["clover_qe"] = cl
     if not args.no_fp64:
-        hb64 = time_hb(qcb, ctx, L, qcb.F64, max(2, args.steps // 2), 1)
+        hb64 = time_hb(qcb, ctx, L, qcb.F64, max(2, args.steps // 2), max(2, args.steps // 2))
         log(f"[bench] HB FP64 {hb64['link_updates_per_s']:.3e} link-updates/s, {hb64['achieved_gbs']:.0f} GB/s")
-        wf64 = time_wf(qcb, ctx, L, qcb.F64, max(2, args.flow_steps // 2), 1)
+        wf64 = time_wf(qcb, ctx, L, qcb.F64, max(2, args.flow_steps // 2), max(2, args.flow_steps // 2))
         log(f"[bench] WF FP64 {wf64['site_steps_per_s']:.3e} site-steps/s, {wf64['achieved_gbs']:.0f} GB/s")
         comp.update(hb_fp64=hb64, wf_fp64=wf64)
         # the plain FP64 GPU configuration: 16^4, hot start from seed 7, N_hits = 2

@@ -69,6 +69,16 @@ struct Req {
     size_t tail_bytes() const { return 16 + 16 * (size_t)(n_steps + 1); }
 };
 
+// Whether a gated launch over a set would update any of its lattices, by the
+// kernels' own test (qcb::gate_open).  A launch whose gates are all closed is
+// skipped: each of its blocks would still look up its lattice's descriptor
+// before returning, ~35 us per launch with a few resident lattices.
+bool any_gate_open(const std::vector<Req*>& set, const Gate& g) {
+    for (const Req* r : set)
+        if (g.mod == 0u || (g.sweep + r->lat.sweep_off) % g.mod == g.res) return true;
+    return false;
+}
+
 // First-fit free list over [0, size) with coalescing.
 struct Arena {
     size_t size = 0;
@@ -470,19 +480,23 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
         const uint32_t nt = (uint32_t)s->n_therm;
         const Launch gl = set_launch(s->set_therm, kFamLink, s->s_hb);
         const Gate first{it, nt ? nt : 1u, 0u}, last{it, nt ? nt : 1u, nt ? nt - 1u : 0u};
-        if (f64) launch_hot<double>(gl, first); else launch_hot<float>(gl, first);
+        if (any_gate_open(s->therm, first)) {
+            if (f64) launch_hot<double>(gl, first); else launch_hot<float>(gl, first);
+        }
         if (nt) {
             const Launch gh = set_launch(s->set_therm, kFamHalf, s->s_hb);
             for (int hs = 0; hs < 8; ++hs) {
                 if (f64) launch_hb_half<double>(gh, hs >> 1, hs & 1, s->n_hits, it, kHbPhilox, nullptr, 0);
                 else launch_hb_half<float>(gh, hs >> 1, hs & 1, s->n_hits, it, kHbPhilox, nullptr, 0);
             }
-            if (s->reunit_every > 0) {
-                const Gate ru{it, (uint32_t)s->reunit_every, (uint32_t)s->reunit_every - 1u};
+            const Gate ru{it, (uint32_t)s->reunit_every, (uint32_t)s->reunit_every - 1u};
+            if (s->reunit_every > 0 && any_gate_open(s->therm, ru)) {
                 if (f64) launch_reunit<double>(gl, ru); else launch_reunit<float>(gl, ru);
             }
         }
-        if (f64) launch_flow_init<double>(gl, last); else launch_flow_init<float>(gl, last);
+        if (any_gate_open(s->therm, last)) {
+            if (f64) launch_flow_init<double>(gl, last); else launch_flow_init<float>(gl, last);
+        }
     }
     QCB_HIP(c, hipEventRecord(s->ev_hb[slot], s->s_hb));
     // 5. flowing lattices: measure every one, step those with steps left; a
