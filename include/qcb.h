@@ -179,11 +179,15 @@ int qcb_batch_hb_sweeps(qcb_batch* b, int n_hits, int n_sweeps, uint32_t sweep_i
 int qcb_batch_reunitarize(qcb_batch* b);
 int qcb_batch_plaquette(qcb_batch* b, double* out /* n */);
 /* Flow all lattices from their links, measuring (t, Q, E_mean) at t = 0, eps,
- * ..., n_steps*eps: tQE[((i*(n_steps+1)) + s)*3 + {0,1,2}]. */
+ * ..., n_steps*eps: tQE[((i*(n_steps+1)) + s)*3 + {0,1,2}].  The launch
+ * sequence is captured into a hipGraph at the first call and replayed by later
+ * calls with the same eps and n_steps (captured again when either, or a buffer
+ * address, changes). */
 int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE);
 /* qcb_flow_steps for every lattice: n RK3 steps of GradientFlow::rk3_step
  * (src/flow/gradient_flow.cpp:117-128) applied to each lattice's own links,
- * one launch per stage for all lattices, no measurement. */
+ * one launch per stage for all lattices, no measurement; replayed from a
+ * captured graph as qcb_batch_flow is. */
 int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps);
 /* Launch schedule of the heatbath: the order of the 8 half-sweeps (hs = 2*mu +
  * parity; any permutation is a valid Gibbs sweep) and a persistent grid size
