@@ -568,22 +568,38 @@ extern "C" int qcb_batch_get_layout(qcb_batch* b, int* n_streams, int tile_caps[
     return QCB_OK;
 }
 
-// Times candidate schedules (half-sweep order x persistent grid) on a scratch
-// copy of every lattice's links and keeps the fastest.  The links themselves
-// are not modified.  report (optional) receives one line per candidate.
+// Times candidate schedules (half-sweep order x persistent grid x stream
+// groups, then work-order tile caps) on a scratch copy of every lattice's
+// links and keeps the fastest.  The links themselves are not modified.
+// report (optional) receives one line per candidate.
+static int autotune_on_scratch(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len);
+
 extern "C" int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len) {
     if (!b || n_hits < 1 || reps < 1) return QCB_EINVAL;
-    qcb_ctx* c = b->ctx;
     int rc;
-    // tune on scratch copies: swap U with F[0] after copying
-    for (qcb_lattice* l : b->lats) {
+    // tune on scratch copies: swap U with F[0] after copying, and swap back on
+    // every return path so the lattices keep their own links
+    for (qcb_lattice* l : b->lats)
         if ((rc = ensure_flow_buffers(l))) return rc;
+    for (qcb_lattice* l : b->lats) {
         Launch one = lat_launch(l, kFamLink);
         if (b->prec == QCB_F64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0]);
         else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0]);
         std::swap(l->U, l->F[0]);
         l->flow_active = false;
     }
+    rc = hipGetLastError() == hipSuccess ? autotune_on_scratch(b, n_hits, reps, report, report_len)
+                                         : qcb_set_error(b->ctx, QCB_EDEVICE, "autotune: copying the links failed");
+    for (qcb_lattice* l : b->lats) std::swap(l->U, l->F[0]);
+    const int rc2 = batch_upload(b);
+    return rc ? rc : rc2;
+}
+
+// The timing part of qcb_batch_autotune, run while every lattice's U is a
+// scratch copy of its links.
+static int autotune_on_scratch(qcb_batch* b, int n_hits, int reps, char* report, size_t report_len) {
+    qcb_ctx* c = b->ctx;
+    int rc;
     if ((rc = batch_upload(b))) return rc;
     int dev = 0, ncu = 0;
     QCB_HIP(c, hipGetDevice(&dev));
@@ -654,8 +670,6 @@ extern "C" int qcb_batch_autotune(qcb_batch* b, int n_hits, int reps, char* repo
     }
     std::memcpy(b->tile_caps, best_t, sizeof b->tile_caps);
     if ((rc = apply_tiles(b))) return rc;
-    for (qcb_lattice* l : b->lats) std::swap(l->U, l->F[0]);
-    if ((rc = batch_upload(b))) return rc;
     if (report && report_len) {
         std::snprintf(report, report_len, "%s", rep.c_str());
     }
