@@ -1,8 +1,8 @@
 """GPU parity: the HIP path through the C-ABI against the FP64 oracle.
 
-Tolerances: FP64 kernels agree with the oracle to 1e-12 relative (the
-north-star bar for staples, plaquette, flow, Q, E and injected-uniform
-heatbath sweeps); Q is compared in absolute terms scaled by E*V because it is
+Tolerances: FP64 kernels agree with the oracle to 1e-12 relative (the bar
+for the deterministic kernels -- staples, plaquette, flow, Q, E -- and for
+heatbath sweeps fed injected uniforms); Q is compared in absolute terms scaled by E*V because it is
 near zero.  FP32 kernels are held to float-rounding bounds stated per test.
 """
 import numpy as np
