@@ -196,18 +196,13 @@ int set_groups(qcb_batch* b, int n) {
 }
 
 // Work-order tiles of every lattice from the caps (largest divisor of each
-// extent not above its cap; Z-order where the tile counts allow it).
+// extent not above its cap), visited y-fastest.
 int apply_tiles(qcb_batch* b) {
     for (qcb_lattice* l : b->lats) {
         int t[4] = {0, divisor_le(l->L[1], b->tile_caps[0]), divisor_le(l->L[2], b->tile_caps[1]),
                     divisor_le(l->L[3], b->tile_caps[2])};
-        int m = 1;
-        for (int d = 1; d < 4; ++d) {
-            const int nt = l->L[d] / t[d];
-            if (nt & (nt - 1)) m = 0;
-        }
         for (int d = 1; d < 4; ++d) l->tile[d] = t[d];
-        l->morton = m;
+        l->morton = 0;   // y-fastest, as default_tiling
         const int rc = upload_desc(l);
         if (rc) return rc;
     }

@@ -86,19 +86,17 @@ inline int divisor_le(int L, int cap) {
     return 1;
 }
 
-// Default tiling: 8 x 4 x 4 sites in (y, z, t) (largest divisors), tiles in
-// Z-order when every tile count is a power of two, else y-fastest.
+// Default tiling: 8 x 4 x 4 sites in (y, z, t) (largest divisors), tiles
+// visited y-fastest.  Z-order over the tiles (qcb_lattice_set_tiling) keeps
+// neighbouring tiles a little closer in time but costs every thread ~54
+// instructions of bit decoding; at 32^4 the y-fastest order is 1.4% faster
+// for the flow stage and 1.3% for the clover, and equal for the heatbath.
 inline void default_tiling(const int L[4], int tile[4], int* morton) {
     tile[0] = 0;
     tile[1] = divisor_le(L[1], 8);
     tile[2] = divisor_le(L[2], 4);
     tile[3] = divisor_le(L[3], 4);
-    int m = 1;
-    for (int d = 1; d < 4; ++d) {
-        const int n = L[d] / tile[d];
-        if (n & (n - 1)) m = 0;
-    }
-    *morton = m;
+    *morton = 0;
 }
 
 // Blocks one lattice contributes to a launch of the given kernel family.

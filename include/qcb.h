@@ -80,7 +80,7 @@ int qcb_lattice_bytes(qcb_lattice* lat, uint64_t* bytes);
  * speed only: every kernel
  * result is independent of it except the last bits of the FP64 global sums,
  * whose block partition follows the tiles.  Default: 8 x 4 x 4 (largest
- * divisors), Z-order when all tile counts are powers of two. */
+ * divisors), y-fastest. */
 int qcb_lattice_set_tiling(qcb_lattice* lat, int ty, int tz, int tt, int morton);
 /* Upload / download of the whole field (src/io/ildg.cpp:87-106,213-232 minus
  * the byte swap, which stays with the caller's file codec). */
