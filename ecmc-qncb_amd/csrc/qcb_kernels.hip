@@ -346,8 +346,15 @@ __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restric
     {   // U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
         M3<T> a = mul_ld<T, false, LAY>(f, S, MU, q, xmnu, ld_link<T, LAY>(f, S, NU, p, xpmumnu));
         M3<T> b = mul_dn(ld_link<T, LAY>(f, S, NU, q, xmnu), a);
-        acc_nd(C, b, ld_link<T, LAY>(f, S, MU, p, i));
-        // C now holds all four leaves (acc_nd adds b * U_mu(x)^dag)
+        if constexpr (LinkSlots<T>::n == 6) {
+            // columns 0, 1 of b U_mu(x)^dag from the stored rows of U_mu(x)
+            M3<T> u = mul_part<T, false, true, 3, 2>(b, ld_link_rows01<LAY>(f, S, MU, p, i));
+            su3_third_col(u);
+            m3_add(C, u);
+        } else {
+            acc_nd(C, b, ld_link<T, LAY>(f, S, MU, p, i));
+        }
+        // C now holds all four leaves
     }
     M3<T> F;
 #pragma unroll
@@ -769,7 +776,8 @@ __device__ __forceinline__ void lds_put_rows01(Vec2<float>::type* a, int hs, uin
 #pragma unroll
     for (int k = 0; k < 6; ++k) a[((uint32_t)hs * 6u + (uint32_t)k) * kLinkSites + s] = {m.v[2 * k], m.v[2 * k + 1]};
 }
-__device__ __forceinline__ M3<float> lds_get_link(const Vec2<float>::type* a, int hs, uint32_t s) {
+// rows 0 and 1 of U_mu(x) from LDS (row 2 unset)
+__device__ __forceinline__ M3<float> lds_get_rows01(const Vec2<float>::type* a, int hs, uint32_t s) {
     M3<float> m;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
@@ -777,7 +785,6 @@ __device__ __forceinline__ M3<float> lds_get_link(const Vec2<float>::type* a, in
         m.v[2 * k] = c.x;
         m.v[2 * k + 1] = c.y;
     }
-    su3_third_row(m);
     return m;
 }
 
@@ -823,14 +830,25 @@ __device__ __forceinline__ void flow_lower(const LatDev<T>& L, const T* __restri
 }
 
 // Sigma += the upper staple through x from its plane's S in LDS and U_nu(x).
+// In FP32 U_nu(x) is its two stored rows: columns 0 and 1 of S U_nu(x)^dag
+// (or S^dag U_nu(x)^dag) need no more, and column 2 of the SU(3) product is
+// rebuilt from them -- 72 + 24 multiply-adds and 18 adds instead of a row-2
+// rebuild of U_nu(x) and 108.
 template <class T, int MU, int NU>
 __device__ __forceinline__ void flow_upper(const T* lds, int p, uint32_t s, const M3<T>& unu, M3<T>& Sg) {
     constexpr int pl = plane_idx(MU < NU ? MU : NU, MU < NU ? NU : MU);
     M3<T> t;
 #pragma unroll
     for (int k = 0; k < 18; ++k) t.v[k] = lds[plane_slot(pl, p, k, s)];
-    if constexpr (MU < NU) acc_nd(Sg, t, unu);
-    else acc_dd(Sg, t, unu);
+    if constexpr (LinkSlots<T>::n == 6) {
+        M3<T> u = mul_part<T, (MU > NU), true, 3, 2>(t, unu);
+        su3_third_col(u);
+        m3_add(Sg, u);
+    } else if constexpr (MU < NU) {
+        acc_nd(Sg, t, unu);
+    } else {
+        acc_dd(Sg, t, unu);
+    }
 }
 
 template <class T, int MU>
@@ -848,12 +866,18 @@ __device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restr
     flow_lower<T, MU, N2>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N2, q, nb_cb(L, c, N2, -1)), Sg);
 }
 
-// U_nu(x) for the update of a link of x: from LDS in FP32, else from the field.
-template <class T>
+// U_nu(x) for the update of a link of x: FP32 rows 0, 1 from LDS (the whole
+// link with ROW2), else the whole link from the field.
+template <class T, bool ROW2 = false>
 __device__ __forceinline__ M3<T> site_link(const LatDev<T>& L, const T* __restrict__ win, const FlowLds<T>& sh,
                                            int nu, int p, uint32_t i, uint32_t s) {
-    if constexpr (LinkSlots<T>::n == 6) return lds_get_link(sh.w, nu * 2 + p, s);
-    else return ld_link<T, kFlowOrder>(win, L.S, nu, p, i);
+    if constexpr (LinkSlots<T>::n == 6) {
+        M3<T> m = lds_get_rows01(sh.w, nu * 2 + p, s);
+        if constexpr (ROW2) su3_third_row(m);
+        return m;
+    } else {
+        return ld_link<T, kFlowOrder>(win, L.S, nu, p, i);
+    }
 }
 
 template <class T, int MU, int STAGE>
@@ -864,7 +888,7 @@ __device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restr
     flow_upper<T, MU, N0>(sh.s, p, s, site_link(L, win, sh, N0, p, i, s), Sg);
     flow_upper<T, MU, N1>(sh.s, p, s, site_link(L, win, sh, N1, p, i, s), Sg);
     flow_upper<T, MU, N2>(sh.s, p, s, site_link(L, win, sh, N2, p, i, s), Sg);
-    flow_tail<T, STAGE>(L, site_link(L, win, sh, MU, p, i, s), wout, zs, MU, p, i, eps, Sg);
+    flow_tail<T, STAGE>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, MU, p, i, eps, Sg);
 }
 
 template <class T, int STAGE>

@@ -352,14 +352,17 @@ __device__ __forceinline__ M3<T> staple(const LatDev<T>& L, const T* __restrict_
         const uint32_t xmumnu = nb_cb(L, c, MU, +1, nu, -1);
         if constexpr (LinkSlots<T>::n == 6) {
             // FP32: the first factor's stored rows give rows 0, 1 of the
-            // product t (an SU(3) matrix), whose row 2 is then rebuilt
+            // product t (an SU(3) matrix), whose row 2 is then rebuilt; the
+            // stored rows of U_nu(x) give columns 0, 1 of t U_nu(x)^dag, whose
+            // column 2 is rebuilt
             {
                 M3<T> a = ld_link_rows01(f, S, nu, q, xmu);   // U_nu(x+mu)
                 M3<T> b = ld_link(f, S, MU, q, xnu);          // U_mu(x+nu)
                 M3<T> t = mul_nd<T, 2>(a, b);
                 su3_third_row(t);
-                M3<T> d = ld_link(f, S, nu, p, i);            // U_nu(x)
-                acc_nd(A, t, d);
+                M3<T> u = mul_part<T, false, true, 3, 2>(t, ld_link_rows01(f, S, nu, p, i));   // U_nu(x)
+                su3_third_col(u);
+                m3_add(A, u);
             }
             {
                 M3<T> a = ld_link_rows01(f, S, MU, q, xmnu);  // U_mu(x-nu)

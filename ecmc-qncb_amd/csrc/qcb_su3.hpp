@@ -224,6 +224,65 @@ __device__ __forceinline__ void acc_dd(M3<T>& acc, const M3<T>& a, const M3<T>& 
         }
 }
 
+// Entries (i, j), i < NR, j < NC, of op(a) op(b), op the Hermitian conjugate
+// where DA / DB is set, the others left unset.  Rows 0, 1 (NR = 2) read only
+// op(a)'s rows 0, 1; columns 0, 1 (NC = 2) only op(b)'s columns 0, 1, which
+// for DB are b's rows 0, 1 -- so a product of SU(3) factors stored as two
+// rows needs none of their row-2 rebuilds when its own third row or column
+// is rebuilt instead (su3_third_row / su3_third_col).
+template <class T, bool DA, bool DB, int NR = 3, int NC = 3>
+__device__ __forceinline__ M3<T> mul_part(const M3<T>& a, const M3<T>& b) {
+    M3<T> c;
+#pragma unroll
+    for (int i = 0; i < NR; ++i)
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            T r = T(0), s = T(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const T ar = DA ? a.re(k, i) : a.re(i, k), ai = DA ? -a.im(k, i) : a.im(i, k);
+                const T br = DB ? b.re(j, k) : b.re(k, j), bi = DB ? -b.im(j, k) : b.im(k, j);
+                if (k == 0) {
+                    r = ar * br;
+                    s = ar * bi;
+                } else {
+                    r = fma(ar, br, r);
+                    s = fma(ar, bi, s);
+                }
+                r = fma(-ai, bi, r);
+                s = fma(ai, br, s);
+            }
+            c.re(i, j) = r;
+            c.im(i, j) = s;
+        }
+    return c;
+}
+
+// Column 2 of an SU(3) matrix from columns 0 and 1: c2 = conj(c0 x c1).
+template <class T>
+__device__ __forceinline__ void su3_third_col(M3<T>& m) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int a = (r + 1) % 3, b = (r + 2) % 3;
+        T re = m.re(a, 0) * m.re(b, 1);
+        re = fma(-m.im(a, 0), m.im(b, 1), re);
+        re = fma(-m.re(b, 0), m.re(a, 1), re);
+        re = fma(m.im(b, 0), m.im(a, 1), re);
+        T im = m.re(b, 0) * m.im(a, 1);
+        im = fma(m.im(b, 0), m.re(a, 1), im);
+        im = fma(-m.re(a, 0), m.im(b, 1), im);
+        im = fma(-m.im(a, 0), m.re(b, 1), im);
+        m.re(r, 2) = re;
+        m.im(r, 2) = im;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ void m3_add(M3<T>& acc, const M3<T>& a) {
+#pragma unroll
+    for (int k = 0; k < 18; ++k) acc.v[k] += a.v[k];
+}
+
 template <class T>
 __device__ __forceinline__ T retrace(const M3<T>& a) {
     return a.v[0] + a.v[8] + a.v[16];
