@@ -713,24 +713,27 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 // Z and Zs are carried as their 9 independent reals (A9).
 template <class T, int STAGE>
 __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T* __restrict__ wout,
-                                          T* __restrict__ zs, int mu, int p, uint32_t i, T eps, const M3<T>& Sg) {
+                                          T* __restrict__ zs, const A9<T>& Z0, int mu, int p, uint32_t i, T eps,
+                                          const M3<T>& Sg) {
     A9<T> Z = force_ta(W, Sg);
     A9<T> X;
     if (STAGE == 1) {
         st_a9(zs, L.S, mu, p, i, Z);
         X = a9_scale(T(0.25) * eps, Z);
     } else if (STAGE == 2) {
-        A9<T> Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
         A9<T> Z1 = a9_axpby(T(8.0 / 9.0), Z, T(-17.0 / 36.0), Z0);
         st_a9(zs, L.S, mu, p, i, Z1);
         X = a9_scale(eps, Z1);
     } else {
-        A9<T> Zc = ld_a9((const T*)zs, L.S, mu, p, i);
-        X = a9_axpby(T(0.75) * eps, Z, -eps, Zc);
+        X = a9_axpby(T(0.75) * eps, Z, -eps, Z0);
     }
     M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
     M3<T> Wn = mul_nn(E, W);
-    su3_store_fix(Wn);
+    // FP32: rows 0, 1 re-orthonormalised once per RK3 step.  The stage-1 and
+    // stage-2 fields are read once, by the next stage, whose row-2 rebuild
+    // then sees rows orthonormal to the rounding of one product; the error
+    // does not carry over from step to step.
+    if (STAGE == 3) su3_store_fix(Wn);
     st_link<T, kFlowOrder>(wout, L.S, mu, p, i, Wn);
 }
 
@@ -882,13 +885,13 @@ __device__ __forceinline__ M3<T> site_link(const LatDev<T>& L, const T* __restri
 
 template <class T, int MU, int STAGE>
 __device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
-                                            T* __restrict__ zs, int p, uint32_t i, uint32_t s, const FlowLds<T>& sh,
-                                            M3<T>& Sg, T eps) {
+                                            T* __restrict__ zs, const A9<T>& Z0, int p, uint32_t i, uint32_t s,
+                                            const FlowLds<T>& sh, M3<T>& Sg, T eps) {
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
     flow_upper<T, MU, N0>(sh.s, p, s, site_link(L, win, sh, N0, p, i, s), Sg);
     flow_upper<T, MU, N1>(sh.s, p, s, site_link(L, win, sh, N1, p, i, s), Sg);
     flow_upper<T, MU, N2>(sh.s, p, s, site_link(L, win, sh, N2, p, i, s), Sg);
-    flow_tail<T, STAGE>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, MU, p, i, eps, Sg);
+    flow_tail<T, STAGE>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, Z0, MU, p, i, eps, Sg);
 }
 
 template <class T, int STAGE>
@@ -911,6 +914,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     T* zs = L.F[2];
     Coords c = tiled_coords(L, live ? q : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
+    // stages 2 and 3 read the force their stage 1 (and 2) stored; its load
+    // is issued first so that its latency hides behind the staples
+    A9<T> Z0;
+    if (STAGE != 1 && live) Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
     M3<T> Sg;
     if (live) {
         switch (mu) {
@@ -923,10 +930,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     __syncthreads();
     if (live) {
         switch (mu) {
-            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
-            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
-            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
-            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, p, i, s, sh, Sg, eps); break;
+            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
+            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
+            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
+            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
         }
     }
 }

@@ -438,11 +438,34 @@ struct InvFact {
     }
 };
 
+// Horner recursion of the Cayley-Hamilton coefficients of exp X truncated
+// after X^N: q(n) = c_n - i d q2, q1 <- q0 - t q2, q2 <- q1 from n = N-1
+// down to 0, starting from q0 = c_N (reference src/su3/utils.cpp:160-185).
+template <class T, int N>
+__device__ __forceinline__ void exp_horner(T t, T d, T& q0r, T& q0i, T& q1r, T& q1i, T& q2r, T& q2i) {
+    constexpr InvFact<T, N> cf;
+    q0r = cf.c[N];
+    q0i = q1r = q1i = q2r = q2i = T(0);
+#pragma unroll
+    for (int n = N - 1; n >= 0; --n) {
+        T n0r = fma(d, q2i, cf.c[n]);
+        T n0i = -d * q2r;
+        T n1r = fma(-t, q2r, q0r);
+        T n1i = fma(-t, q2i, q0i);
+        q2r = q1r;
+        q2i = q1i;
+        q1r = n1r;
+        q1i = n1i;
+        q0r = n0r;
+        q0i = n0i;
+    }
+}
+
 // exp(X) for X in su(3), given by its 9 reals, by Luescher's Cayley-Hamilton
 // Horner scheme (reference src/su3/utils.cpp:145-193): with t = -Re Tr X^2/2
 // and d = Re(i det X), q(n) = c_n - i d q2, q1 <- q0 - t q2, q2 <- q1 from
-// n = N-1 down to 0, and exp X = q0 + q1 X + q2 X^2.  N = 18 in double; float
-// needs fewer terms for the same relative accuracy (||X|| <= 1 in the flow).
+// n = N-1 down to 0, and exp X = q0 + q1 X + q2 X^2.  N = 18 in double; in
+// float N = 0 lets each wavefront take the fewest terms its largest |X| allows.
 // X^2 is Hermitian, so only its upper triangle and real diagonal are
 // multiplied out, and the zero real diagonal of X is never multiplied.
 template <class T, int N>
@@ -489,20 +512,23 @@ __device__ __forceinline__ M3<T> exp_su3_ah(const A9<T>& x) {
     T c2i = X.re(1, 0) * X.im(2, 1) + X.im(1, 0) * X.re(2, 1) - x11 * X.re(2, 0);
     T det_im = x00 * c0r + X.re(0, 1) * c1i + X.im(0, 1) * c1r + X.re(0, 2) * c2i + X.im(0, 2) * c2r;
     T d = -det_im;
-    constexpr InvFact<T, N> cf;
-    T q0r = cf.c[N], q0i = T(0), q1r = T(0), q1i = T(0), q2r = T(0), q2i = T(0);
-#pragma unroll
-    for (int n = N - 1; n >= 0; --n) {
-        T n0r = fma(d, q2i, cf.c[n]);
-        T n0i = -d * q2r;
-        T n1r = fma(-t, q2r, q0r);
-        T n1i = fma(-t, q2i, q0i);
-        q2r = q1r;
-        q2i = q1i;
-        q1r = n1r;
-        q1i = n1i;
-        q0r = n0r;
-        q0i = n0i;
+    T q0r, q0i, q1r, q1i, q2r, q2i;
+    if constexpr (N > 0) {
+        exp_horner<T, N>(t, d, q0r, q0i, q1r, q1i, q2r, q2i);
+    } else {
+        // FP32 term count per wavefront: the Taylor remainder after X^n is at
+        // most |X|^(n+1)/(n+1)! / (1 - |X|/(n+2)) in any submultiplicative
+        // norm; with the Frobenius norm, |X|_F^2 = 2t, it stays below 2^-26
+        // for n = 5 when t <= 0.01096 (|X|_F <= 0.1481) and for n = 8 when
+        // t <= 0.1546 (|X|_F <= 0.556); otherwise 12 terms (|X|_F up to ~3).
+        // The flow's X = eps * (combination of forces) is ~0.01 |Z| at
+        // eps = 0.02.
+        if (__all(t <= T(0.01096)))
+            exp_horner<T, 5>(t, d, q0r, q0i, q1r, q1i, q2r, q2i);
+        else if (__all(t <= T(0.1546)))
+            exp_horner<T, 8>(t, d, q0r, q0i, q1r, q1i, q2r, q2i);
+        else
+            exp_horner<T, 12>(t, d, q0r, q0i, q1r, q1i, q2r, q2i);
     }
     M3<T> e;  // q0 + q1 X + q2 Y
 #pragma unroll
@@ -523,7 +549,7 @@ __device__ __forceinline__ M3<T> exp_su3_ah(const A9<T>& x) {
 
 template <class T> struct ExpTerms;
 template <> struct ExpTerms<double> { static constexpr int N = 18; };
-template <> struct ExpTerms<float> { static constexpr int N = 12; };
+template <> struct ExpTerms<float> { static constexpr int N = 0; };   // chosen per wavefront (5, 8 or 12)
 
 // Gram-Schmidt on columns, c2 = conj(c0 x c1), all in double
 // (reference src/gauge/GaugeField.cpp:36-58).

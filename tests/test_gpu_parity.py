@@ -123,6 +123,27 @@ def test_probe_exp_proj_reunit(ctx):
         assert np.abs(qo.c3(x) - b).max() < 1e-14
 
 
+def test_probe_exp_fp32_term_tiers(ctx):
+    """The FP32 exponential takes 5, 8 or 12 Taylor terms per wavefront, by
+    the largest |X|_F of its lanes (Taylor remainder below 2^-26 for
+    |X|_F <= 0.148, 0.556 and ~3).  One wavefront per call, every lane at the
+    same norm, so each call runs one tier: |X|_F = 0.05 and 0.14 (5 terms),
+    0.3 and 0.55 (8), 1.0 and 2.5 (12).  Each result is unitary to float
+    rounding and equals scipy's expm to 2e-6 (5e-6 at |X|_F = 2.5, where
+    the coefficient recursion accumulates more rounding)."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(5)
+    Z = _su3_algebra(rng, 64, 1.0)
+    Z /= np.linalg.norm(Z, axis=(1, 2))[:, None, None]
+    flat = Z.view(np.float64).reshape(64, 18)
+    for norm, tol in ((0.05, 2e-6), (0.14, 2e-6), (0.3, 2e-6), (0.55, 2e-6), (1.0, 2e-6), (2.5, 5e-6)):
+        out = ctx.probe(qcb.F32, qcb.PROBE_EXP, flat, norm).view(np.complex128).reshape(64, 3, 3)
+        want = np.stack([expm(norm * z) for z in Z])
+        assert np.abs(out - want).max() < tol, norm
+        uu = np.einsum("nij,nkj->nik", out, out.conj())
+        assert np.abs(uu - np.eye(3)).max() < 2e-6, norm
+
+
 def _kp_inputs(seed, n, K):
     rng = np.random.default_rng(seed)
     return np.concatenate([rng.uniform(0.05, 3.0, size=(n, 1)), rng.uniform(size=(n, 4 * K + 2))], 1)
