@@ -67,8 +67,9 @@ namespace qcb {
 inline uint64_t field_bytes(const qcb_lattice* l) {
     return (uint64_t)8 * (l->esize == 4 ? 12 : 18) * l->S * (uint64_t)l->esize;
 }
-// Bytes of the flow's force accumulator: 5 complex slots (9 reals used) per link.
-inline uint64_t force_bytes(const qcb_lattice* l) { return (uint64_t)8 * 10 * l->S * (uint64_t)l->esize; }
+// Bytes of the flow's force accumulator: 8 reals per link (a traceless
+// anti-Hermitian matrix).
+inline uint64_t force_bytes(const qcb_lattice* l) { return (uint64_t)8 * 8 * l->S * (uint64_t)l->esize; }
 // Flow buffers: two link fields (W double buffer) and the force accumulator.
 inline uint64_t flow_bytes(const qcb_lattice* l) { return 2 * field_bytes(l) + force_bytes(l); }
 

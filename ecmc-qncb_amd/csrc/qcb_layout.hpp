@@ -276,42 +276,57 @@ __device__ __forceinline__ void st_link(T* f, uint32_t S, int mu, int p, uint32_
     }
 }
 
-// Force fields hold traceless anti-Hermitian matrices as 5 complex slots per
-// link (A9 in qcb_su3.hpp), in the same AoSoA-64 order as the links.
+// Force fields hold traceless anti-Hermitian matrices (A9 in qcb_su3.hpp) by
+// their 8 independent reals: the upper triangle (o0..o5) and the imaginary
+// diagonal entries d0, d1; d2 = -(d0 + d1) on load.  They sit in 16-byte
+// slots in the links' AoSoA-64 order, (o0 o1 o2 o3)(o4 o5 d0 d1) in FP32 and
+// (o0 o1)(o2 o3)(o4 o5)(d0 d1) in FP64: 32 / 64 bytes per link, two / four
+// loads.
+template <class T>
+struct ForceSlot {
+    typedef T type __attribute__((ext_vector_type(16 / sizeof(T))));
+    static constexpr int per = 16 / (int)sizeof(T), n = 8 / per;   // reals per slot, slots per link
+};
+
+template <class T>
 __device__ __forceinline__ size_t a9_base(uint32_t S, int hs, uint32_t i) {
-    return ((size_t)hs * (S >> 6) + (i >> 6)) * 320u + (i & 63u);
+    return ((size_t)hs * (S >> 6) + (i >> 6)) * (64u * ForceSlot<T>::n) + (i & 63u);
 }
 
 template <class T>
 __device__ __forceinline__ A9<T> ld_a9(const T* f, uint32_t S, int mu, int p, uint32_t i) {
-    typedef typename Vec2<T>::type V2;
-    const QCB_GLOBAL V2* b = (const QCB_GLOBAL V2*)f + a9_base(S, mu * 2 + p, i);
+    typedef ForceSlot<T> FS;
+    typedef typename FS::type V;
+    const QCB_GLOBAL V* b = (const QCB_GLOBAL V*)f + a9_base<T>(S, mu * 2 + p, i);
+    T r[8];
+#pragma unroll
+    for (int j = 0; j < FS::n; ++j) {
+        const V c = b[64 * j];
+#pragma unroll
+        for (int e = 0; e < FS::per; ++e) r[FS::per * j + e] = c[e];
+    }
     A9<T> z;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        V2 c = b[64 * k];
-        z.o[2 * k] = c.x;
-        z.o[2 * k + 1] = c.y;
-    }
-    V2 c3 = b[64 * 3], c4 = b[64 * 4];
-    z.d[0] = c3.x;
-    z.d[1] = c3.y;
-    z.d[2] = c4.x;
+    for (int k = 0; k < 6; ++k) z.o[k] = r[k];
+    z.d[0] = r[6];
+    z.d[1] = r[7];
+    z.d[2] = -(r[6] + r[7]);
     return z;
 }
 
 template <class T>
 __device__ __forceinline__ void st_a9(T* f, uint32_t S, int mu, int p, uint32_t i, const A9<T>& z) {
-    typedef typename Vec2<T>::type V2;
-    QCB_GLOBAL V2* b = (QCB_GLOBAL V2*)f + a9_base(S, mu * 2 + p, i);
+    typedef ForceSlot<T> FS;
+    typedef typename FS::type V;
+    QCB_GLOBAL V* b = (QCB_GLOBAL V*)f + a9_base<T>(S, mu * 2 + p, i);
+    const T r[8] = {z.o[0], z.o[1], z.o[2], z.o[3], z.o[4], z.o[5], z.d[0], z.d[1]};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        V2 v = {z.o[2 * k], z.o[2 * k + 1]};
-        __builtin_nontemporal_store(v, b + 64 * k);
+    for (int j = 0; j < FS::n; ++j) {
+        V v;
+#pragma unroll
+        for (int e = 0; e < FS::per; ++e) v[e] = r[FS::per * j + e];
+        __builtin_nontemporal_store(v, b + 64 * j);
     }
-    V2 v3 = {z.d[0], z.d[1]}, v4 = {z.d[2], T(0)};
-    __builtin_nontemporal_store(v3, b + 64 * 3);
-    __builtin_nontemporal_store(v4, b + 64 * 4);
 }
 
 // Block index -> work tile such that each XCD owns one contiguous range of

@@ -332,11 +332,11 @@ __device__ __forceinline__ void proj_lie(M3<T>& a) {
     a = b;
 }
 
-// A traceless anti-Hermitian matrix by its 9 independent reals (the
-// reference keeps all 18, src/flow/gradient_flow.cpp:15-34):
+// A traceless anti-Hermitian matrix by 9 reals (the reference keeps all 18,
+// src/flow/gradient_flow.cpp:15-34):
 //   o = {Re Z01, Im Z01, Re Z02, Im Z02, Re Z12, Im Z12},  d = {Im Z00, Im Z11, Im Z22},
-// Z10 = -conj(Z01) etc., Re Z_ii = 0.  Memory form: 5 complex slots
-// {Z01, Z02, Z12, (d0, d1), (d2, 0)}.
+// Z10 = -conj(Z01) etc., Re Z_ii = 0, d0 + d1 + d2 = 0.  Memory form (ld_a9 /
+// st_a9): the 8 independent reals, d2 rebuilt as -(d0 + d1).
 template <class T>
 struct A9 {
     T o[6];
