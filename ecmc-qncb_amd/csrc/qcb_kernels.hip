@@ -711,11 +711,12 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 //  stage 2: Zs = 8/9 Z - 17/36 Zs,       W' = exp(eps Zs) W
 //  stage 3:                              W' = exp(eps (3/4 Z - Zs)) W
 // Z and Zs are carried as their 9 independent reals (A9).
-template <class T, int STAGE>
+template <class T, int STAGE, bool ADD>
 __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T* __restrict__ wout,
                                           T* __restrict__ zs, const A9<T>& Z0, int mu, int p, uint32_t i, T eps,
-                                          const M3<T>& Sg) {
+                                          const M3<T>& Sg, const A9<T>& Zx) {
     A9<T> Z = force_ta(W, Sg);
+    if constexpr (ADD) Z = a9_axpby(T(1), Z, T(1), Zx);
     A9<T> X;
     if (STAGE == 1) {
         st_a9(zs, L.S, mu, p, i, Z);
@@ -742,19 +743,27 @@ __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T*
 // (a < b) the product S = U_b(x+a) U_a(x+b)^dag is the start of both upper
 // staples through x,
 //   Sigma_a += S U_b(x)^dag,   Sigma_b += S^dag U_a(x)^dag,
-// so one of the two threads forms it (owners spread 2/1/2/1 over the four
-// wavefronts) and hands it to the other through LDS.  The upper staples of
-// the four links of x also share the links U_nu(x) themselves; these kernels
-// are bound by their loads more than by their arithmetic (an FP32 stage with
-// all nine complex entries stored, which skips every row-2 rebuild, runs 37%
-// slower), so in FP32 each thread publishes its own link in LDS and the other
-// three read it there.  The lower staples U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b)
-// are not shared (publishing their common U_b(x-b) as well costs a block of
-// residency per CU and gained nothing).  Phase 1: own link (FP32) and owned S
-// into LDS, then the three lower staples; barrier; phase 2: the three upper
-// staples and the update.
+// so one of the two threads forms it and hands it to the other through LDS.
+// The upper staples of the four links of x also share the links U_nu(x)
+// themselves; in FP32 each thread publishes its own link in LDS and the other
+// three read it there.  A block's 32 sites are whole x rows whenever Lx/2
+// divides 32 (the work order runs x fastest over whole rows), and then the
+// lower staple of U_mu(x), mu != 0, in the plane (0, mu) comes from the block
+// too: with y = x - e0, the x-neighbour in the same row,
+//   U_mu(x) U_0(y+mu)^dag U_mu(y)^dag U_0(y) = S_(0,mu)(y) U_mu(y)^dag U_0(y),
+// S_(0,mu)(y) = U_mu(y+e0) U_0(y+mu)^dag being plane (0, mu)'s shared product
+// at y, so it enters the force as P_TA(-S U_mu(y)^dag U_0(y)) from LDS
+// (no loads, two products) instead of a three-link staple.  The other lower
+// staples U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b) are formed from the field.
+// Phase 1: own link (FP32) and owned S into LDS, then the lower staples from
+// the field; barrier; phase 2: the upper staples, the (mu, 0) lower staple
+// from LDS where the rows are whole, and the update.  The S owners balance
+// phase 1: 2/1/2/1 over mu = 0..3 when mu = 0 forms three lower staples and
+// the others two, 0/2/2/2 when the others form two and mu = 0 three.
 __device__ __forceinline__ constexpr int plane_idx(int a, int b) { return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5); }
+template <bool XL>
 __device__ __forceinline__ constexpr int plane_owner(int pl) {
+    if constexpr (XL) return pl == 0 ? 1 : pl == 1 ? 2 : pl == 2 ? 3 : pl == 3 ? 1 : pl == 4 ? 3 : 2;
     return pl == 0 ? 0 : pl == 1 ? 2 : pl == 2 ? 0 : pl == 3 ? 1 : pl == 4 ? 3 : 2;
 }
 // LDS slot of entry k of plane pl's S at parity p, site s of the block
@@ -792,12 +801,12 @@ __device__ __forceinline__ M3<float> lds_get_rows01(const Vec2<float>::type* a, 
 }
 
 // The owned plane's upper-staple start S = U_b(x+a) U_a(x+b)^dag into LDS.
-template <class T, int MU, int NU>
+template <class T, int MU, int NU, bool XL>
 __device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
                                                uint32_t s, T* lds) {
     constexpr int a = MU < NU ? MU : NU, b = MU < NU ? NU : MU;
     constexpr int pl = plane_idx(a, b);
-    if constexpr (plane_owner(pl) == MU) {
+    if constexpr (plane_owner<XL>(pl) == MU) {
         const int q = p ^ 1;
         const uint32_t xa = nb_cb(L, c, a, +1), xb = nb_cb(L, c, b, +1);
         M3<T> t;
@@ -854,19 +863,41 @@ __device__ __forceinline__ void flow_upper(const T* lds, int p, uint32_t s, cons
     }
 }
 
-template <class T, int MU>
+template <class T, int MU, bool XL>
 __device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
                                             uint32_t s, uint32_t i, FlowLds<T>& sh, M3<T>& Sg) {
     if constexpr (LinkSlots<T>::n == 6) lds_put_rows01(sh.w, MU * 2 + p, s, ld_link_rows01<kFlowOrder>(win, L.S, MU, p, i));
     m3_zero(Sg);
     const int q = p ^ 1;
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
-    flow_put_plane<T, MU, N0>(L, win, c, p, s, sh.s);
-    flow_lower<T, MU, N0>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N0, q, nb_cb(L, c, N0, -1)), Sg);
-    flow_put_plane<T, MU, N1>(L, win, c, p, s, sh.s);
+    flow_put_plane<T, MU, N0, XL>(L, win, c, p, s, sh.s);
+    if constexpr (!XL || N0 != 0)   // with XL the (mu, 0) lower staple comes from LDS in phase 2
+        flow_lower<T, MU, N0>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N0, q, nb_cb(L, c, N0, -1)), Sg);
+    flow_put_plane<T, MU, N1, XL>(L, win, c, p, s, sh.s);
     flow_lower<T, MU, N1>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N1, q, nb_cb(L, c, N1, -1)), Sg);
-    flow_put_plane<T, MU, N2>(L, win, c, p, s, sh.s);
+    flow_put_plane<T, MU, N2, XL>(L, win, c, p, s, sh.s);
     flow_lower<T, MU, N2>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N2, q, nb_cb(L, c, N2, -1)), Sg);
+}
+
+// FP32, mu != 0, whole x rows in the block: the force term of the lower
+// staple of U_mu(x) in the plane (mu, 0), P_TA(-S_(0,mu)(y) U_mu(y)^dag U_0(y))
+// with y = x - e0, entirely from LDS (S and the stored rows of the links of
+// y, whose slot is s plus the cb step to y: the row's slots are contiguous).
+template <int MU>
+__device__ __forceinline__ A9<float> flow_xlower(const LatDev<float>& L, Coords c, int p, uint32_t s,
+                                                 const FlowLds<float>& sh) {
+    static_assert(MU != 0, "the (0, 0) plane does not exist");
+    const uint32_t sy = s + cb_step(L, c, 0, -1);
+    const int q = p ^ 1;
+    constexpr int pl = plane_idx(0, MU);
+    M3<float> t;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) t.v[k] = sh.s[plane_slot(pl, q, k, sy)];
+    M3<float> r = mul_part<float, false, true, 3, 2>(t, lds_get_rows01(sh.w, MU * 2 + q, sy));   // S U_mu(y)^dag
+    su3_third_col(r);
+    M3<float> u0 = lds_get_rows01(sh.w, q, sy);   // U_0(y)
+    su3_third_row(u0);
+    return force_ta(r, u0);
 }
 
 // U_nu(x) for the update of a link of x: FP32 rows 0, 1 from LDS (the whole
@@ -883,15 +914,46 @@ __device__ __forceinline__ M3<T> site_link(const LatDev<T>& L, const T* __restri
     }
 }
 
-template <class T, int MU, int STAGE>
+template <class T, int MU, int STAGE, bool XL>
 __device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
-                                            T* __restrict__ zs, const A9<T>& Z0, int p, uint32_t i, uint32_t s,
-                                            const FlowLds<T>& sh, M3<T>& Sg, T eps) {
+                                            T* __restrict__ zs, const A9<T>& Z0, Coords c, int p, uint32_t i,
+                                            uint32_t s, const FlowLds<T>& sh, M3<T>& Sg, T eps) {
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
     flow_upper<T, MU, N0>(sh.s, p, s, site_link(L, win, sh, N0, p, i, s), Sg);
     flow_upper<T, MU, N1>(sh.s, p, s, site_link(L, win, sh, N1, p, i, s), Sg);
     flow_upper<T, MU, N2>(sh.s, p, s, site_link(L, win, sh, N2, p, i, s), Sg);
-    flow_tail<T, STAGE>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, Z0, MU, p, i, eps, Sg);
+    constexpr bool ADD = XL && MU != 0;
+    A9<T> Zx;
+    if constexpr (ADD) Zx = flow_xlower<MU>(L, c, p, s, sh);
+    flow_tail<T, STAGE, ADD>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, Z0, MU, p, i, eps, Sg, Zx);
+}
+
+template <class T, int STAGE, bool XL>
+__device__ __forceinline__ void flow_stage_body(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
+                                                T* __restrict__ zs, bool live, int mu, Coords c, int p, uint32_t i,
+                                                uint32_t s, FlowLds<T>& sh, T eps) {
+    // stages 2 and 3 read the force their stage 1 (and 2) stored; its load
+    // is issued first so that its latency hides behind the staples
+    A9<T> Z0;
+    if (STAGE != 1 && live) Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
+    M3<T> Sg;
+    if (live) {
+        switch (mu) {
+            case 0: flow_phase1<T, 0, XL>(L, win, c, p, s, i, sh, Sg); break;
+            case 1: flow_phase1<T, 1, XL>(L, win, c, p, s, i, sh, Sg); break;
+            case 2: flow_phase1<T, 2, XL>(L, win, c, p, s, i, sh, Sg); break;
+            default: flow_phase1<T, 3, XL>(L, win, c, p, s, i, sh, Sg); break;
+        }
+    }
+    __syncthreads();
+    if (live) {
+        switch (mu) {
+            case 0: flow_phase2<T, 0, STAGE, XL>(L, win, wout, zs, Z0, c, p, i, s, sh, Sg, eps); break;
+            case 1: flow_phase2<T, 1, STAGE, XL>(L, win, wout, zs, Z0, c, p, i, s, sh, Sg, eps); break;
+            case 2: flow_phase2<T, 2, STAGE, XL>(L, win, wout, zs, Z0, c, p, i, s, sh, Sg, eps); break;
+            default: flow_phase2<T, 3, STAGE, XL>(L, win, wout, zs, Z0, c, p, i, s, sh, Sg, eps); break;
+        }
+    }
 }
 
 template <class T, int STAGE>
@@ -903,7 +965,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     if (gstep - L.flow_start >= L.flow_limit) return;   // the whole block: its lattice has finished flowing
     const uint32_t s = threadIdx.x & (kLinkSites - 1);
     const uint32_t q = bc.local * kLinkSites + s;
-    const uint32_t hs = threadIdx.x / kLinkSites;
+    // wavefront w of the block takes direction (w + block) mod 4, so that
+    // each SIMD runs every direction's share of the work
+    const uint32_t hs = (((threadIdx.x >> 6) + blockIdx.x) & 3u) * 2u + ((threadIdx.x >> 5) & 1u);
     // every thread reaches the barrier; a site past the end has all four of
     // its links past the end, so no live thread reads what it did not write
     const bool live = q < L.Vh;
@@ -914,28 +978,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     T* zs = L.F[2];
     Coords c = tiled_coords(L, live ? q : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-    // stages 2 and 3 read the force their stage 1 (and 2) stored; its load
-    // is issued first so that its latency hides behind the staples
-    A9<T> Z0;
-    if (STAGE != 1 && live) Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
-    M3<T> Sg;
-    if (live) {
-        switch (mu) {
-            case 0: flow_phase1<T, 0>(L, win, c, p, s, i, sh, Sg); break;
-            case 1: flow_phase1<T, 1>(L, win, c, p, s, i, sh, Sg); break;
-            case 2: flow_phase1<T, 2>(L, win, c, p, s, i, sh, Sg); break;
-            default: flow_phase1<T, 3>(L, win, c, p, s, i, sh, Sg); break;
-        }
-    }
-    __syncthreads();
-    if (live) {
-        switch (mu) {
-            case 0: flow_phase2<T, 0, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
-            case 1: flow_phase2<T, 1, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
-            case 2: flow_phase2<T, 2, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
-            default: flow_phase2<T, 3, STAGE>(L, win, wout, zs, Z0, p, i, s, sh, Sg, eps); break;
-        }
-    }
+    // whole x rows per block (Lx/2 divides the block's 32 sites): the
+    // (mu, 0) lower staples come from LDS; the branch is uniform per block
+    if (LinkSlots<T>::n == 6 && (kLinkSites % (uint32_t)L.Lxh) == 0u)
+        flow_stage_body<T, STAGE, LinkSlots<T>::n == 6>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
+    else
+        flow_stage_body<T, STAGE, false>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
 }
 
 // ---------------------------------------------------------- math probes ----
