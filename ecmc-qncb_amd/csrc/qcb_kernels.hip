@@ -801,24 +801,36 @@ __device__ __forceinline__ M3<float> lds_get_rows01(const Vec2<float>::type* a, 
 }
 
 // The owned plane's upper-staple start S = U_b(x+a) U_a(x+b)^dag into LDS.
+// With whole x rows in the block the owner of a plane (0, b) is b, and
+// U_b(x+e0) is the link that the other half of its own wavefront (the other
+// parity, same direction) has just published in LDS: LDS operations of one
+// wavefront complete in order, so a wavefront fence that keeps the compiler
+// from moving the read above the publishing write is all it needs.
 template <class T, int MU, int NU, bool XL>
 __device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
-                                               uint32_t s, T* lds) {
+                                               uint32_t s, FlowLds<T>& sh) {
     constexpr int a = MU < NU ? MU : NU, b = MU < NU ? NU : MU;
     constexpr int pl = plane_idx(a, b);
     if constexpr (plane_owner<XL>(pl) == MU) {
         const int q = p ^ 1;
-        const uint32_t xa = nb_cb(L, c, a, +1), xb = nb_cb(L, c, b, +1);
+        const uint32_t xb = nb_cb(L, c, b, +1);
         M3<T> t;
         if constexpr (LinkSlots<T>::n == 6) {
-            M3<T> u = ld_link_rows01<kFlowOrder>(win, L.S, b, q, xa);
+            M3<T> u;
+            if constexpr (XL && a == 0) {
+                static_assert(b == MU, "planes (0, b) are owned by b on whole rows");
+                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                u = lds_get_rows01(sh.w, MU * 2 + q, s + cb_step(L, c, 0, +1));   // U_b(x+e0)
+            } else {
+                u = ld_link_rows01<kFlowOrder>(win, L.S, b, q, nb_cb(L, c, a, +1));
+            }
             t = mul_nd<T, 2>(u, ld_link<T, kFlowOrder>(win, L.S, a, q, xb));
             su3_third_row(t);
         } else {
-            t = mul_nd(ld_link<T, kFlowOrder>(win, L.S, b, q, xa), ld_link<T, kFlowOrder>(win, L.S, a, q, xb));
+            t = mul_nd(ld_link<T, kFlowOrder>(win, L.S, b, q, nb_cb(L, c, a, +1)), ld_link<T, kFlowOrder>(win, L.S, a, q, xb));
         }
 #pragma unroll
-        for (int k = 0; k < 18; ++k) lds[plane_slot(pl, p, k, s)] = t.v[k];
+        for (int k = 0; k < 18; ++k) sh.s[plane_slot(pl, p, k, s)] = t.v[k];
     }
 }
 
@@ -870,12 +882,12 @@ __device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restr
     m3_zero(Sg);
     const int q = p ^ 1;
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
-    flow_put_plane<T, MU, N0, XL>(L, win, c, p, s, sh.s);
+    flow_put_plane<T, MU, N0, XL>(L, win, c, p, s, sh);
     if constexpr (!XL || N0 != 0)   // with XL the (mu, 0) lower staple comes from LDS in phase 2
         flow_lower<T, MU, N0>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N0, q, nb_cb(L, c, N0, -1)), Sg);
-    flow_put_plane<T, MU, N1, XL>(L, win, c, p, s, sh.s);
+    flow_put_plane<T, MU, N1, XL>(L, win, c, p, s, sh);
     flow_lower<T, MU, N1>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N1, q, nb_cb(L, c, N1, -1)), Sg);
-    flow_put_plane<T, MU, N2, XL>(L, win, c, p, s, sh.s);
+    flow_put_plane<T, MU, N2, XL>(L, win, c, p, s, sh);
     flow_lower<T, MU, N2>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N2, q, nb_cb(L, c, N2, -1)), Sg);
 }
 
