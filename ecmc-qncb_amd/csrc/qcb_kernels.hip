@@ -752,14 +752,16 @@ __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T*
 // too: with y = x - e0, the x-neighbour in the same row,
 //   U_mu(x) U_0(y+mu)^dag U_mu(y)^dag U_0(y) = S_(0,mu)(y) U_mu(y)^dag U_0(y),
 // S_(0,mu)(y) = U_mu(y+e0) U_0(y+mu)^dag being plane (0, mu)'s shared product
-// at y, so it enters the force as P_TA(-S U_mu(y)^dag U_0(y)) from LDS
-// (no loads, two products) instead of a three-link staple.  The other lower
-// staples U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b) are formed from the field.
+// at y, so it enters the force as P_TA(-S U_mu(y)^dag U_0(y)) with S from
+// LDS (in FP32 the links of y too: no loads; in FP64 two links from the
+// field) instead of a three-link staple.  The other lower staples
+// U_b(x+a-b)^dag U_a(x-b)^dag U_b(x-b) are formed from the field.
 // Phase 1: own link (FP32) and owned S into LDS, then the lower staples from
 // the field; barrier; phase 2: the upper staples, the (mu, 0) lower staple
 // from LDS where the rows are whole, and the update.  The S owners balance
-// phase 1: 2/1/2/1 over mu = 0..3 when mu = 0 forms three lower staples and
-// the others two, 0/2/2/2 when the others form two and mu = 0 three.
+// phase 1: 2/1/2/1 over mu = 0..3 when every direction forms its three lower
+// staples from the field, 0/2/2/2 when mu = 1..3 take their (mu, 0) lower
+// staple from LDS and form two while mu = 0 forms three.
 __device__ __forceinline__ constexpr int plane_idx(int a, int b) { return a == 0 ? b - 1 : (a == 1 ? b + 1 : 5); }
 template <bool XL>
 __device__ __forceinline__ constexpr int plane_owner(int pl) {
@@ -817,7 +819,7 @@ __device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __re
         M3<T> t;
         if constexpr (LinkSlots<T>::n == 6) {
             M3<T> u;
-            if constexpr (XL && a == 0) {
+            if constexpr (XL && a == 0) {   // FP32 only: the stored rows of the block's links are in LDS
                 static_assert(b == MU, "planes (0, b) are owned by b on whole rows");
                 __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                 u = lds_get_rows01(sh.w, MU * 2 + q, s + cb_step(L, c, 0, +1));   // U_b(x+e0)
@@ -891,25 +893,32 @@ __device__ __forceinline__ void flow_phase1(const LatDev<T>& L, const T* __restr
     flow_lower<T, MU, N2>(L, win, c, p, ld_link<T, kFlowOrder>(win, L.S, N2, q, nb_cb(L, c, N2, -1)), Sg);
 }
 
-// FP32, mu != 0, whole x rows in the block: the force term of the lower
-// staple of U_mu(x) in the plane (mu, 0), P_TA(-S_(0,mu)(y) U_mu(y)^dag U_0(y))
-// with y = x - e0, entirely from LDS (S and the stored rows of the links of
-// y, whose slot is s plus the cb step to y: the row's slots are contiguous).
-template <int MU>
-__device__ __forceinline__ A9<float> flow_xlower(const LatDev<float>& L, Coords c, int p, uint32_t s,
-                                                 const FlowLds<float>& sh) {
+// mu != 0, whole x rows in the block: the force term of the lower staple of
+// U_mu(x) in the plane (mu, 0), P_TA(-S_(0,mu)(y) U_mu(y)^dag U_0(y)) with
+// y = x - e0: S from LDS (the row's slots are contiguous, so y's slot is s
+// plus the cb step to y), the two links of y from LDS in FP32 (their stored
+// rows) and from the field in FP64.
+template <class T, int MU>
+__device__ __forceinline__ A9<T> flow_xlower(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
+                                             uint32_t i, uint32_t s, const FlowLds<T>& sh) {
     static_assert(MU != 0, "the (0, 0) plane does not exist");
-    const uint32_t sy = s + cb_step(L, c, 0, -1);
+    const uint32_t step = cb_step(L, c, 0, -1);
+    const uint32_t sy = s + step;
     const int q = p ^ 1;
     constexpr int pl = plane_idx(0, MU);
-    M3<float> t;
+    M3<T> t;
 #pragma unroll
     for (int k = 0; k < 18; ++k) t.v[k] = sh.s[plane_slot(pl, q, k, sy)];
-    M3<float> r = mul_part<float, false, true, 3, 2>(t, lds_get_rows01(sh.w, MU * 2 + q, sy));   // S U_mu(y)^dag
-    su3_third_col(r);
-    M3<float> u0 = lds_get_rows01(sh.w, q, sy);   // U_0(y)
-    su3_third_row(u0);
-    return force_ta(r, u0);
+    if constexpr (LinkSlots<T>::n == 6) {
+        M3<T> r = mul_part<T, false, true, 3, 2>(t, lds_get_rows01(sh.w, MU * 2 + q, sy));   // S U_mu(y)^dag
+        su3_third_col(r);
+        M3<T> u0 = lds_get_rows01(sh.w, q, sy);   // U_0(y)
+        su3_third_row(u0);
+        return force_ta(r, u0);
+    } else {
+        const uint32_t iy = i + step;
+        return force_ta(mul_nd(t, ld_link<T, kFlowOrder>(win, L.S, MU, q, iy)), ld_link<T, kFlowOrder>(win, L.S, 0, q, iy));
+    }
 }
 
 // U_nu(x) for the update of a link of x: FP32 rows 0, 1 from LDS (the whole
@@ -936,7 +945,7 @@ __device__ __forceinline__ void flow_phase2(const LatDev<T>& L, const T* __restr
     flow_upper<T, MU, N2>(sh.s, p, s, site_link(L, win, sh, N2, p, i, s), Sg);
     constexpr bool ADD = XL && MU != 0;
     A9<T> Zx;
-    if constexpr (ADD) Zx = flow_xlower<MU>(L, c, p, s, sh);
+    if constexpr (ADD) Zx = flow_xlower<T, MU>(L, win, c, p, i, s, sh);
     flow_tail<T, STAGE, ADD>(L, site_link<T, true>(L, win, sh, MU, p, i, s), wout, zs, Z0, MU, p, i, eps, Sg, Zx);
 }
 
@@ -992,8 +1001,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     // whole x rows per block (Lx/2 divides the block's 32 sites): the
     // (mu, 0) lower staples come from LDS; the branch is uniform per block
-    if (LinkSlots<T>::n == 6 && (kLinkSites % (uint32_t)L.Lxh) == 0u)
-        flow_stage_body<T, STAGE, LinkSlots<T>::n == 6>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
+    if ((kLinkSites % (uint32_t)L.Lxh) == 0u)
+        flow_stage_body<T, STAGE, true>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
     else
         flow_stage_body<T, STAGE, false>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
 }
