@@ -315,6 +315,92 @@ __device__ __forceinline__ M3<T> mul_ld(const T* __restrict__ f, uint32_t S, int
     }
 }
 
+// Leaves 1 and 4 of plane (MU, NU) at x (reference leaf order
+// src/observables/observables_mpi.cpp:73-81), the two through x+mu, for the
+// FP32 clover's x-planes (k_clover); clover_F below forms all four.
+template <class T, int LAY>
+struct CloverLeaves {
+    const LatDev<T>& L;
+    const T* __restrict__ f;
+    Coords c;
+    int p, MU, NU;
+    uint32_t i;
+    // Q1 = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag
+    __device__ __forceinline__ M3<T> q1() const {
+        const int q = p ^ 1;
+        M3<T> a = mul_ld<T, false, LAY>(f, L.S, MU, p, i, ld_link<T, LAY>(f, L.S, NU, q, nb_cb(L, c, MU, +1)));
+        M3<T> b = mul_ld<T, false, LAY>(f, L.S, NU, p, i, ld_link<T, LAY>(f, L.S, MU, q, nb_cb(L, c, NU, +1)));
+        return mul_nd(a, b);
+    }
+    // C += Q4 = U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
+    __device__ __forceinline__ void q4(M3<T>& C) const {
+        const int q = p ^ 1;
+        const uint32_t xmnu = nb_cb(L, c, NU, -1);
+        M3<T> a = mul_ld<T, false, LAY>(f, L.S, MU, q, xmnu, ld_link<T, LAY>(f, L.S, NU, p, nb_cb(L, c, MU, +1, NU, -1)));
+        M3<T> b = mul_dn(ld_link<T, LAY>(f, L.S, NU, q, xmnu), a);
+        if constexpr (LinkSlots<T>::n == 6) {
+            // columns 0, 1 of b U_mu(x)^dag from the stored rows of U_mu(x)
+            M3<T> u = mul_part<T, false, true, 3, 2>(b, ld_link_rows01<LAY>(f, L.S, MU, p, i));
+            su3_third_col(u);
+            m3_add(C, u);
+        } else {
+            acc_nd(C, b, ld_link<T, LAY>(f, L.S, MU, p, i));
+        }
+    }
+};
+
+// C - C^dag of a sum of leaves C, by its 9 reals (anti-Hermitian; not made
+// traceless, as the reference's F is not)
+template <class T>
+__device__ __forceinline__ A9<T> ah_part(const M3<T>& C) {
+    A9<T> h;
+    constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        h.o[2 * e] = C.re(pi[e], pj[e]) - C.re(pj[e], pi[e]);
+        h.o[2 * e + 1] = C.im(pi[e], pj[e]) + C.im(pj[e], pi[e]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) h.d[k] = T(2) * C.im(k, k);
+    return h;
+}
+
+// V^dag H V for an anti-Hermitian H given by its 9 reals: the result is
+// anti-Hermitian, so only its upper triangle and imaginary diagonal are formed
+template <class T>
+__device__ __forceinline__ A9<T> ah_conj(const M3<T>& V, const A9<T>& h) {
+    const M3<T> B = mul_nn(a9_full(h), V);
+    A9<T> r;
+    constexpr int pi[3] = {0, 0, 1}, pj[3] = {1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const int a = pi[e], b = pj[e];
+        T re = T(0), im = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {   // sum_k conj(V_ka) B_kb
+            re = fma(V.re(k, a), B.re(k, b), re);
+            re = fma(V.im(k, a), B.im(k, b), re);
+            im = fma(V.re(k, a), B.im(k, b), im);
+            im = fma(-V.im(k, a), B.re(k, b), im);
+        }
+        r.o[2 * e] = re;
+        r.o[2 * e + 1] = im;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        T im = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            im = fma(V.re(k, a), B.im(k, a), im);
+            im = fma(-V.im(k, a), B.re(k, a), im);
+        }
+        r.d[a] = im;
+    }
+    return r;
+}
+
+// F_mu,nu(x) = (C - C^dag) / 8 with C the sum of the four leaves
+// (reference src/observables/observables_mpi.cpp:61-85).
 template <class T, int LAY>
 __device__ __forceinline__ M3<T> clover_F(const LatDev<T>& L, const T* __restrict__ f, Coords c, int p, uint32_t i,
                                           const int MU, const int NU) {
@@ -377,20 +463,62 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 
 // q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
 // (reference src/observables/observables_mpi.cpp:88-107).
+//
+// In FP32 a wavefront holds 32 cb sites of both parities (lanes 0-31 parity
+// 0, 32-63 parity 1, the same 32 cb sites).  They are whole x rows whenever
+// Lx/2 divides 32 (the work order runs x fastest over whole rows), and then
+// the x-neighbour y = x - e0 of every site is in the same wavefront.  In a
+// plane (0, nu) two of x's leaves are y's leaves conjugated by V = U_0(y):
+// Q2(x) = V^dag Q1(y) V and Q3(x) = V^dag Q4(y) V, so with
+// H = (Q1 + Q4) - (Q1 + Q4)^dag, the anti-Hermitian part that F needs,
+//   8 F_0nu(x) = H(x) + V^dag H(y) V.
+// Each site forms Q1 + Q4 of its three x-planes, publishes H in LDS and adds
+// its neighbour's conjugated H (LDS operations of one wavefront complete in
+// order: a wavefront fence orders the read after the neighbour's write).  6
+// leaves of the 24 are not formed (their 18 products and ~15 link loads),
+// for three conjugations.  The other planes' four leaves are formed as they
+// stand.
 template <class T, int LAY>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::clover))) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                    int n_lat, uint32_t nblocks, uint32_t gstep,
                                                    double* __restrict__ pq, double* __restrict__ pe) {
     __shared__ double lds[kBlock / 64];
+    __shared__ T hx[LinkSlots<T>::n == 6 ? 2 * 3 * 9 * kSiteSites : 1];   // H of the x-planes: [(parity * 3 + nu - 1) * 9 + k][site]
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    const uint32_t qs = bc.local * kSiteSites + (threadIdx.x & (kSiteSites - 1));
-    const int p = (int)(threadIdx.x / kSiteSites);
+    // FP32 only: the FP64 kernel holds one wave per SIMD and is bound by its
+    // L2 misses, and the shared leaves made it 1-2% slower (more spills)
+    constexpr bool kShare = LinkSlots<T>::n == 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t s = kShare ? (threadIdx.x >> 6) * 32u + (lane & 31u) : threadIdx.x & (kSiteSites - 1);
+    const uint32_t qs = bc.local * kSiteSites + s;
+    const int p = (int)(kShare ? lane >> 5 : threadIdx.x / kSiteSites);
+    const bool live = qs < L.Vh;   // rows are whole, so a live site's y is live
+    const bool xrows = kShare && (32u % (uint32_t)L.Lxh) == 0u;   // uniform per block
+    const Coords c = tiled_coords(L, live ? qs : 0u, p);
+    const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
+    const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
+    if (xrows) {
+        if (live) {
+#pragma unroll 1
+            for (int nu = 1; nu < 4; ++nu) {
+                const CloverLeaves<T, LAY> lv{L, f, c, p, 0, nu, i};
+                M3<T> C = lv.q1();
+                __builtin_amdgcn_sched_barrier(0);
+                lv.q4(C);
+                const A9<T> h = ah_part(C);
+                T* dst = hx + (uint32_t)((p * 3 + nu - 1) * 9) * kSiteSites + s;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) dst[k * kSiteSites] = h.o[k];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) dst[(6 + k) * kSiteSites] = h.d[k];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
     double q = 0.0, e = 0.0;
-    if (qs < L.Vh) {
-        Coords c = tiled_coords(L, qs, p);
-        const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-        const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
+    if (live) {
+        const uint32_t step = cb_step(L, c, 0, -1);   // cb index and block slot of y = x - e0
         double tq = 0.0;
         // the three plane pairs (01,23), (02,13), (03,12) one at a time: a
         // loop the compiler keeps rolled bounds the registers to one pair
@@ -398,8 +526,21 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         for (int pr = 0; pr < 3; ++pr) {
             const int nu = pr + 1;                            // (0, nu) and its complement
             const int m2 = pr == 2 ? 1 : (pr == 1 ? 1 : 2), n2 = pr == 0 ? 3 : (pr == 1 ? 3 : 2);
-            M3<T> a = clover_F<T, LAY>(L, f, c, p, i, 0, nu);
-            M3<T> b = clover_F<T, LAY>(L, f, c, p, i, m2, n2);
+            M3<T> a;
+            if (xrows) {
+                const T* own = hx + (uint32_t)((p * 3 + nu - 1) * 9) * kSiteSites + s;
+                const T* nbr = hx + (uint32_t)(((p ^ 1) * 3 + nu - 1) * 9) * kSiteSites + (s + step);
+                A9<T> hy, h;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) { hy.o[k] = nbr[k * kSiteSites]; h.o[k] = own[k * kSiteSites]; }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) { hy.d[k] = nbr[(6 + k) * kSiteSites]; h.d[k] = own[(6 + k) * kSiteSites]; }
+                const A9<T> g = ah_conj(ld_link<T, LAY>(f, L.S, 0, p ^ 1, i + step), hy);
+                a = a9_full(a9_axpby(T(0.125), h, T(0.125), g));
+            } else {
+                a = clover_F<T, LAY>(L, f, c, p, i, 0, nu);
+            }
+            const M3<T> b = clover_F<T, LAY>(L, f, c, p, i, m2, n2);
             const double tr = (double)retrace_nn(a, b);
             tq += pr == 1 ? -tr : tr;
             e += sqnorm(a) + sqnorm(b);

@@ -5,8 +5,8 @@
 // t_max with a measurement before every RK3 step and after the last one.
 // Requests wait in a FIFO queue and are admitted (first fit, later requests
 // may pass a blocked head) when their footprint fits in a fixed HBM arena:
-// four link-sized fields, the request's exhausted-draw counter and its
-// (Q, E) series.  Every service iteration advances every resident request by
+// the links, the flow's two link fields and its force field, the request's
+// exhausted-draw counter and its (Q, E) series.  Every service iteration advances every resident request by
 // one unit of work in batched launches: one heatbath sweep for each
 // thermalising lattice on one stream, one measurement + RK3 step for each
 // flowing lattice on a second stream, so new work joins the device while

@@ -173,6 +173,32 @@ def test_fp32_flow_links_against_fp64(ctx, dims):
     s.close()
 
 
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (16, 8, 8, 8), (64, 4, 4, 4), (12, 6, 6, 8), (24, 4, 4, 6)])
+def test_fp32_clover_against_fp64(ctx, dims):
+    """The FP32 clover forms leaves 2 and 3 of the planes (0, nu) as the
+    x-neighbour's leaves 1 and 4 conjugated by U_0(x - e0) when a wavefront's
+    32 sites are whole x rows (Lx/2 divides 32: Lx = 8, 16, 64), and all four
+    leaves from the field otherwise (Lx = 12, 24).  Both against the FP64
+    clover of the same links, on the gauge field and on a flowed field:
+    |dQ| < 2e-6 sqrt(V) and |dE|/E < 1e-6 (float rounding; a wrong leaf
+    would be off at O(1))."""
+    d = thermal_links(ctx, dims, sweeps=3, seed=9)
+    s = qcb.Lattice(ctx, dims, qcb.F32)
+    s.set_links(d.get_links())
+    V = float(np.prod(dims))
+    for step in range(2):
+        if step:
+            for g in (d, s):
+                g.flow_copy()
+                g.flow_rk3(0.02, 5)
+        q64, e64 = d.flow_qe() if step else d.clover_qe()
+        q32, e32 = s.flow_qe() if step else s.clover_qe()
+        assert abs(q32 - q64) < 2e-6 * np.sqrt(V), (dims, step, q32, q64)
+        assert abs(e32 - e64) / e64 < 1e-6, (dims, step, e32, e64)
+    d.close()
+    s.close()
+
+
 def _binned(x, nbin=20):
     x = np.asarray(x)
     bins = x[: len(x) // nbin * nbin].reshape(nbin, -1).mean(axis=1)
