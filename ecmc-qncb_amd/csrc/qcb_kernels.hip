@@ -317,7 +317,7 @@ __device__ __forceinline__ M3<T> mul_ld(const T* __restrict__ f, uint32_t S, int
 
 // Leaves 1 and 4 of plane (MU, NU) at x (reference leaf order
 // src/observables/observables_mpi.cpp:73-81), the two through x+mu, for the
-// FP32 clover's x-planes (k_clover); clover_F below forms all four.
+// clover's x-planes (k_clover); clover_F below forms all four.
 template <class T, int LAY>
 struct CloverLeaves {
     const LatDev<T>& L;
@@ -464,8 +464,8 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 // q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
 // (reference src/observables/observables_mpi.cpp:88-107).
 //
-// In FP32 a wavefront holds 32 cb sites of both parities (lanes 0-31 parity
-// 0, 32-63 parity 1, the same 32 cb sites).  They are whole x rows whenever
+// A wavefront holds 32 cb sites of both parities (lanes 0-31 parity 0,
+// 32-63 parity 1, the same 32 cb sites).  They are whole x rows whenever
 // Lx/2 divides 32 (the work order runs x fastest over whole rows), and then
 // the x-neighbour y = x - e0 of every site is in the same wavefront.  In a
 // plane (0, nu) two of x's leaves are y's leaves conjugated by V = U_0(y):
@@ -483,18 +483,15 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
                                                    int n_lat, uint32_t nblocks, uint32_t gstep,
                                                    double* __restrict__ pq, double* __restrict__ pe) {
     __shared__ double lds[kBlock / 64];
-    __shared__ T hx[LinkSlots<T>::n == 6 ? 2 * 3 * 9 * kSiteSites : 1];   // H of the x-planes: [(parity * 3 + nu - 1) * 9 + k][site]
+    __shared__ T hx[2 * 3 * 9 * kSiteSites];   // H of the x-planes: [(parity * 3 + nu - 1) * 9 + k][site]
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
-    // FP32 only: the FP64 kernel holds one wave per SIMD and is bound by its
-    // L2 misses, and the shared leaves made it 1-2% slower (more spills)
-    constexpr bool kShare = LinkSlots<T>::n == 6;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t s = kShare ? (threadIdx.x >> 6) * 32u + (lane & 31u) : threadIdx.x & (kSiteSites - 1);
+    const uint32_t s = (threadIdx.x >> 6) * 32u + (lane & 31u);
     const uint32_t qs = bc.local * kSiteSites + s;
-    const int p = (int)(kShare ? lane >> 5 : threadIdx.x / kSiteSites);
+    const int p = (int)(lane >> 5);
     const bool live = qs < L.Vh;   // rows are whole, so a live site's y is live
-    const bool xrows = kShare && (32u % (uint32_t)L.Lxh) == 0u;   // uniform per block
+    const bool xrows = (32u % (uint32_t)L.Lxh) == 0u;   // uniform per block
     const Coords c = tiled_coords(L, live ? qs : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
