@@ -55,8 +55,9 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
 
 // Occupancy floor of the heavy kernels: with no floor the scheduler hoists
 // every neighbour load and the double kernels fall to one wave per SIMD.  The
-// clover is the exception: FP32 at 3 waves spills 31-41 VGPRs to scratch and
-// runs 14% slower than at 2 (236 VGPRs); FP64 runs fastest at 1 (512 VGPRs).
+// clover is the exception: with a floor of 2 the FP32 kernel takes 146 VGPRs
+// and runs 3 waves per SIMD (a floor of 4 spills to scratch and is 15%
+// slower); FP64 runs fastest at 1 (256 VGPRs, no scratch; at 2 it spills).
 #ifndef QCB_HB_WAVES_F32
 #define QCB_HB_WAVES_F32 3
 #endif
