@@ -34,6 +34,11 @@ __device__ __forceinline__ BlockCtx<T> block_ctx(const LatDev<T>* __restrict__ l
 }
 
 // --- block reduction of doubles (wave64 shuffles, then LDS across 4 waves) ---
+// Makes this wavefront's earlier LDS writes visible to its own later LDS
+// reads from other lanes: every LDS operation of the wavefront has
+// completed (lgkmcnt 0) and the compiler moves no memory access across.
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -474,8 +479,9 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 // H = (Q1 + Q4) - (Q1 + Q4)^dag, the anti-Hermitian part that F needs,
 //   8 F_0nu(x) = H(x) + V^dag H(y) V.
 // Each site forms Q1 + Q4 of its three x-planes, publishes H in LDS and adds
-// its neighbour's conjugated H (LDS operations of one wavefront complete in
-// order: a wavefront fence orders the read after the neighbour's write).  6
+// its neighbour's conjugated H (the neighbour is a lane of the same
+// wavefront: waiting for the wavefront's LDS writes, wave_lds_sync, is all
+// the read needs).  6
 // leaves of the 24 are not formed (their 18 products and ~15 link loads),
 // for three conjugations.  The other planes' four leaves are formed as they
 // stand.
@@ -512,7 +518,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
                 for (int k = 0; k < 3; ++k) dst[(6 + k) * kSiteSites] = h.d[k];
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        wave_lds_sync();
     }
     double q = 0.0, e = 0.0;
     if (live) {
@@ -953,9 +959,8 @@ __device__ __forceinline__ M3<float> lds_get_rows01(const Vec2<float>::type* a, 
 // The owned plane's upper-staple start S = U_b(x+a) U_a(x+b)^dag into LDS.
 // With whole x rows in the block the owner of a plane (0, b) is b, and
 // U_b(x+e0) is the link that the other half of its own wavefront (the other
-// parity, same direction) has just published in LDS: LDS operations of one
-// wavefront complete in order, so a wavefront fence that keeps the compiler
-// from moving the read above the publishing write is all it needs.
+// parity, same direction) has just published in LDS, so waiting for the
+// wavefront's LDS writes (wave_lds_sync) is all the read needs.
 template <class T, int MU, int NU, bool XL>
 __device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __restrict__ win, Coords c, int p,
                                                uint32_t s, FlowLds<T>& sh) {
@@ -969,7 +974,7 @@ __device__ __forceinline__ void flow_put_plane(const LatDev<T>& L, const T* __re
             M3<T> u;
             if constexpr (XL && a == 0) {   // FP32 only: the stored rows of the block's links are in LDS
                 static_assert(b == MU, "planes (0, b) are owned by b on whole rows");
-                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                wave_lds_sync();
                 u = lds_get_rows01(sh.w, MU * 2 + q, s + cb_step(L, c, 0, +1));   // U_b(x+e0)
             } else {
                 u = ld_link_rows01<kFlowOrder>(win, L.S, b, q, nb_cb(L, c, a, +1));
