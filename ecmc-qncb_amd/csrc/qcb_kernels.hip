@@ -321,9 +321,9 @@ __device__ __forceinline__ M3<T> mul_ld(const T* __restrict__ f, uint32_t S, int
     }
 }
 
-// Leaves 1 and 4 of plane (MU, NU) at x (reference leaf order
-// src/observables/observables_mpi.cpp:73-81), the two through x+mu, for the
-// clover's x-planes (k_clover); clover_F below forms all four.
+// The leaves of plane (MU, NU) at x one at a time (reference leaf order
+// src/observables/observables_mpi.cpp:73-81), for the clover's shared planes
+// (k_clover); clover_F below forms all four in one piece.
 template <class T, int LAY>
 struct CloverLeaves {
     const LatDev<T>& L;
@@ -337,6 +337,22 @@ struct CloverLeaves {
         M3<T> a = mul_ld<T, false, LAY>(f, L.S, MU, p, i, ld_link<T, LAY>(f, L.S, NU, q, nb_cb(L, c, MU, +1)));
         M3<T> b = mul_ld<T, false, LAY>(f, L.S, NU, p, i, ld_link<T, LAY>(f, L.S, MU, q, nb_cb(L, c, NU, +1)));
         return mul_nd(a, b);
+    }
+    // C += Q2 = U_nu(x) U_mu(x-mu+nu)^dag U_nu(x-mu)^dag U_mu(x-mu)
+    __device__ __forceinline__ void q2(M3<T>& C) const {
+        const int q = p ^ 1;
+        const uint32_t xmmu = nb_cb(L, c, MU, -1);
+        M3<T> a = mul_ld<T, true, LAY>(f, L.S, NU, p, i, ld_link<T, LAY>(f, L.S, MU, p, nb_cb(L, c, MU, -1, NU, +1)));
+        M3<T> b = mul_dn(ld_link<T, LAY>(f, L.S, MU, q, xmmu), ld_link<T, LAY>(f, L.S, NU, q, xmmu));
+        acc_nd(C, a, b);
+    }
+    // C += Q3 = U_mu(x-mu)^dag U_nu(x-mu-nu)^dag U_mu(x-mu-nu) U_nu(x-nu)
+    __device__ __forceinline__ void q3(M3<T>& C) const {
+        const int q = p ^ 1;
+        const uint32_t xmmumnu = nb_cb(L, c, MU, -1, NU, -1);
+        M3<T> a = mul_ld<T, false, LAY>(f, L.S, NU, p, xmmumnu, ld_link<T, LAY>(f, L.S, MU, q, nb_cb(L, c, MU, -1)));
+        M3<T> b = mul_ld<T, false, LAY>(f, L.S, MU, p, xmmumnu, ld_link<T, LAY>(f, L.S, NU, q, nb_cb(L, c, NU, -1)));
+        acc_dn(C, a, b);
     }
     // C += Q4 = U_nu(x-nu)^dag U_mu(x-nu) U_nu(x+mu-nu) U_mu(x)^dag
     __device__ __forceinline__ void q4(M3<T>& C) const {
@@ -470,59 +486,83 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 // q_x = Re[Tr F01F23 - Tr F02F13 + Tr F03F12]/(4 pi^2), e_x = 1/2 sum ||F||^2
 // (reference src/observables/observables_mpi.cpp:88-107).
 //
+// In a plane (mu, nu), mu < nu, two of x's leaves are the leaves of its
+// neighbour y = x - mu conjugated by V = U_mu(y): Q2(x) = V^dag Q1(y) V and
+// Q3(x) = V^dag Q4(y) V, so with H = (Q1 + Q4) - (Q1 + Q4)^dag, the
+// anti-Hermitian part that F needs,
+//   8 F_mu,nu(x) = H(x) + V^dag H(x - mu) V.
 // A wavefront holds 32 cb sites of both parities (lanes 0-31 parity 0,
-// 32-63 parity 1, the same 32 cb sites).  They are whole x rows whenever
-// Lx/2 divides 32 (the work order runs x fastest over whole rows), and then
-// the x-neighbour y = x - e0 of every site is in the same wavefront.  In a
-// plane (0, nu) two of x's leaves are y's leaves conjugated by V = U_0(y):
-// Q2(x) = V^dag Q1(y) V and Q3(x) = V^dag Q4(y) V, so with
-// H = (Q1 + Q4) - (Q1 + Q4)^dag, the anti-Hermitian part that F needs,
-//   8 F_0nu(x) = H(x) + V^dag H(y) V.
-// Each site forms Q1 + Q4 of its three x-planes, publishes H in LDS and adds
-// its neighbour's conjugated H (the neighbour is a lane of the same
-// wavefront: waiting for the wavefront's LDS writes, wave_lds_sync, is all
-// the read needs).  6
-// leaves of the 24 are not formed (their 18 products and ~15 link loads),
-// for three conjugations.  The other planes' four leaves are formed as they
-// stand.
+// 32-63 parity 1, the same 32 cb sites), and a block 128 cb sites.  They
+// are whole x rows whenever Lx/2 divides 32 (the work order runs x fastest
+// over whole rows), and consecutive y rows of a tile follow each other.
+// Every site then forms Q1 + Q4 of the planes (0, nu) and (1, 2), (1, 3) and
+// publishes their H in LDS; after a barrier a site takes F of the planes
+// (0, nu) from its x-neighbour's H (always in the block), and F of (1, 2),
+// (1, 3) from its y-neighbour's H when every site of its wavefront has that
+// neighbour in the block (checked against the cb indices the block
+// publishes), else from its own leaves 2 and 3.  Plane (2, 3) forms its four
+// leaves.  Of the 24 leaves 14 are formed (18 in the wavefronts without
+// their y-neighbours), for five (three) conjugations.
 template <class T, int LAY>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves<T>::clover))) void k_clover(const LatDev<T>* __restrict__ lats, const uint32_t* __restrict__ blk,
                                                    int n_lat, uint32_t nblocks, uint32_t gstep,
                                                    double* __restrict__ pq, double* __restrict__ pe) {
+    constexpr int kShared = 5;   // planes with H in LDS: (0,1), (0,2), (0,3), (1,2), (1,3)
     __shared__ double lds[kBlock / 64];
-    __shared__ T hx[2 * 3 * 9 * kSiteSites];   // H of the x-planes: [(parity * 3 + nu - 1) * 9 + k][site]
+    __shared__ T hx[2 * kShared * 9 * kSiteSites];   // [(parity * 5 + plane) * 9 + k][site]
+    __shared__ uint32_t cbs[kSiteSites];             // cb index of each slot (both parities share it)
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t s = (threadIdx.x >> 6) * 32u + (lane & 31u);
     const uint32_t qs = bc.local * kSiteSites + s;
     const int p = (int)(lane >> 5);
-    const bool live = qs < L.Vh;   // rows are whole, so a live site's y is live
+    const bool live = qs < L.Vh;   // rows are whole, so a live site's x-neighbour is live
     const bool xrows = (32u % (uint32_t)L.Lxh) == 0u;   // uniform per block
     const Coords c = tiled_coords(L, live ? qs : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
+    constexpr int pmu[kShared] = {0, 0, 0, 1, 1}, pnu[kShared] = {1, 2, 3, 2, 3};
     if (xrows) {
         if (live) {
+            if (p == 0) cbs[s] = i;
 #pragma unroll 1
-            for (int nu = 1; nu < 4; ++nu) {
-                const CloverLeaves<T, LAY> lv{L, f, c, p, 0, nu, i};
+            for (int k5 = 0; k5 < kShared; ++k5) {
+                const CloverLeaves<T, LAY> lv{L, f, c, p, pmu[k5], pnu[k5], i};
                 M3<T> C = lv.q1();
                 __builtin_amdgcn_sched_barrier(0);
                 lv.q4(C);
                 const A9<T> h = ah_part(C);
-                T* dst = hx + (uint32_t)((p * 3 + nu - 1) * 9) * kSiteSites + s;
+                T* dst = hx + (uint32_t)((p * kShared + k5) * 9) * kSiteSites + s;
 #pragma unroll
                 for (int k = 0; k < 6; ++k) dst[k * kSiteSites] = h.o[k];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) dst[(6 + k) * kSiteSites] = h.d[k];
             }
         }
-        wave_lds_sync();
+        __syncthreads();
     }
+    // F of shared plane k5 from H(x) and the neighbour's H at slot sn (cb in):
+    // (H(x) + V^dag H(x - mu) V) / 8, V = U_mu(x - mu)
+    auto shared_F = [&](int k5, uint32_t sn, uint32_t in) {
+        const T* own = hx + (uint32_t)((p * kShared + k5) * 9) * kSiteSites + s;
+        const T* nbr = hx + (uint32_t)(((p ^ 1) * kShared + k5) * 9) * kSiteSites + sn;
+        A9<T> hy, h;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) { hy.o[k] = nbr[k * kSiteSites]; h.o[k] = own[k * kSiteSites]; }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { hy.d[k] = nbr[(6 + k) * kSiteSites]; h.d[k] = own[(6 + k) * kSiteSites]; }
+        const A9<T> g = ah_conj(ld_link<T, LAY>(f, L.S, pmu[k5], p ^ 1, in), hy);
+        return a9_full(a9_axpby(T(0.125), h, T(0.125), g));
+    };
     double q = 0.0, e = 0.0;
     if (live) {
-        const uint32_t step = cb_step(L, c, 0, -1);   // cb index and block slot of y = x - e0
+        const uint32_t step0 = cb_step(L, c, 0, -1);   // cb index and block slot of x - e0
+        const uint32_t step1 = cb_step(L, c, 1, -1);   // cb index step to x - e1
+        // x - e1 in the block for every site of the wavefront (its slot is s - Lx/2)
+        const uint32_t sy = s - (uint32_t)L.Lxh;
+        const bool yok = xrows && s >= (uint32_t)L.Lxh && cbs[s >= (uint32_t)L.Lxh ? sy : s] == i + step1;
+        const bool yshare = __all(yok);
         double tq = 0.0;
         // the three plane pairs (01,23), (02,13), (03,12) one at a time: a
         // loop the compiler keeps rolled bounds the registers to one pair
@@ -530,21 +570,31 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         for (int pr = 0; pr < 3; ++pr) {
             const int nu = pr + 1;                            // (0, nu) and its complement
             const int m2 = pr == 2 ? 1 : (pr == 1 ? 1 : 2), n2 = pr == 0 ? 3 : (pr == 1 ? 3 : 2);
-            M3<T> a;
-            if (xrows) {
-                const T* own = hx + (uint32_t)((p * 3 + nu - 1) * 9) * kSiteSites + s;
-                const T* nbr = hx + (uint32_t)(((p ^ 1) * 3 + nu - 1) * 9) * kSiteSites + (s + step);
-                A9<T> hy, h;
+            const M3<T> a = xrows ? shared_F(nu - 1, s + step0, i + step0) : clover_F<T, LAY>(L, f, c, p, i, 0, nu);
+            M3<T> b;
+            if (xrows && pr > 0) {   // (1, 3) or (1, 2): H(x) in LDS
+                const int k5 = pr == 1 ? 4 : 3;
+                if (yshare) {
+                    b = shared_F(k5, sy, i + step1);
+                } else {
+                    const CloverLeaves<T, LAY> lv{L, f, c, p, m2, n2, i};
+                    M3<T> C;
+                    m3_zero(C);
+                    lv.q2(C);
+                    __builtin_amdgcn_sched_barrier(0);
+                    lv.q3(C);
+                    const A9<T> h23 = ah_part(C);
+                    const T* own = hx + (uint32_t)((p * kShared + k5) * 9) * kSiteSites + s;
+                    A9<T> h;
 #pragma unroll
-                for (int k = 0; k < 6; ++k) { hy.o[k] = nbr[k * kSiteSites]; h.o[k] = own[k * kSiteSites]; }
+                    for (int k = 0; k < 6; ++k) h.o[k] = own[k * kSiteSites];
 #pragma unroll
-                for (int k = 0; k < 3; ++k) { hy.d[k] = nbr[(6 + k) * kSiteSites]; h.d[k] = own[(6 + k) * kSiteSites]; }
-                const A9<T> g = ah_conj(ld_link<T, LAY>(f, L.S, 0, p ^ 1, i + step), hy);
-                a = a9_full(a9_axpby(T(0.125), h, T(0.125), g));
+                    for (int k = 0; k < 3; ++k) h.d[k] = own[(6 + k) * kSiteSites];
+                    b = a9_full(a9_axpby(T(0.125), h, T(0.125), h23));
+                }
             } else {
-                a = clover_F<T, LAY>(L, f, c, p, i, 0, nu);
+                b = clover_F<T, LAY>(L, f, c, p, i, m2, n2);
             }
-            const M3<T> b = clover_F<T, LAY>(L, f, c, p, i, m2, n2);
             const double tr = (double)retrace_nn(a, b);
             tq += pr == 1 ? -tr : tr;
             e += sqnorm(a) + sqnorm(b);

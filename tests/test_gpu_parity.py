@@ -59,7 +59,9 @@ def test_hot_start_matches_oracle(ctx, dims):
     assert np.abs(np.linalg.det(U) - 1).max() < 1e-13
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 6, 8, 6)])
+# (32, 8, 4, 4): the clover's wavefronts hold two whole x rows and most take
+# their y-planes' leaves 2 and 3 from the y-neighbour (8^4: x-planes only)
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (4, 6, 8, 6), (32, 8, 4, 4)])
 def test_plaquette_clover_staples(ctx, dims):
     g, o = hot_pair(ctx, dims, 11)
     # a few oracle sweeps make a non-trivial, partly ordered field
@@ -357,7 +359,9 @@ def test_reunitarize_and_translate(ctx):
     assert np.array_equal(g.get_links(), o.links)
 
 
-@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (10, 6, 4, 8)])
+# (32, 8, 4, 4): whole x rows per block, so the flow takes its (mu, 0) lower
+# staples and the clover its x- and y-planes' shared leaves from LDS
+@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (10, 6, 4, 8), (32, 8, 4, 4)])
 def test_flow_matches_oracle(ctx, dims):
     g, o = hot_pair(ctx, dims, 17)
     for s in range(2):
