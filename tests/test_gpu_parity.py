@@ -85,6 +85,30 @@ def test_plaquette_clover_staples(ctx, dims):
     f.close()
 
 
+# Lx/2 in {2, 4, 8, 16, 32} (whole x rows per wavefront) with y tiles of
+# 4, 6 and 8 rows: wavefronts with and without all their y-neighbours in the
+# block, and lattices where no wavefront has them (Lx = 12: no sharing)
+@pytest.mark.parametrize("dims", [(4, 8, 4, 6), (8, 4, 8, 4), (16, 6, 4, 4), (32, 4, 6, 4), (64, 4, 4, 4),
+                                  (12, 8, 4, 4)])
+def test_clover_shared_leaves_match_oracle(ctx, dims):
+    """The clover's shared leaves (x-planes from the x-neighbour, y-planes
+    from the y-neighbour) against the oracle's four-leaf clover, FP64 to
+    1e-12, on the links and on a flowed field."""
+    g, o = hot_pair(ctx, dims, 19)
+    for s in range(2):
+        o.sweep_philox(6.0, 1, 5, s)
+    g.set_links(o.links)
+    Qg, Eg = g.clover_qe()
+    Qo, Eo = o.clover_qe()
+    assert abs(Eg - Eo) < REL * Eo
+    assert abs(Qg - Qo) < REL * Eo * g.V
+    tg = g.topo_charge_flowed(0.02, 3, 1)
+    to = o.topo_flowed(0.02, 3, 1)
+    assert np.all(np.abs(tg[:, 2] - to[:, 2]) < REL * to[:, 2])
+    assert np.all(np.abs(tg[:, 1] - to[:, 1]) < REL * to[:, 2] * g.V)
+    g.close()
+
+
 def test_abelian_flux_closed_form(ctx):
     from test_oracle_lattice import abelian_field
     dims = (6, 4, 4, 8)
