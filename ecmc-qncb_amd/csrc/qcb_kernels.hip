@@ -900,10 +900,12 @@ __global__ void k_bump(uint32_t* __restrict__ counter, uint32_t n) {
 
 // --------------------------------------------------------------- flow ------
 
-// Stages 2 and 3 read the force that stage 1 (and 2) stored.  FP64 issues
-// that load first, so that its latency hides behind the staples (the FP64
-// stage is bound by load latency: 1.5% faster); in FP32 the load waits for
-// the tail, where it costs fewer live registers (1.5% faster than early).
+// Stages 2 and 3 read the force that stage 1 (and 2) stored, Z0 below.  FP64
+// issues that load first, so that its latency hides behind the staples (the
+// FP64 stage is bound by load latency: 1.5% faster); FP32 issues it right
+// after the block barrier, where the phase-1 temporaries are dead and the
+// upper staples still cover it (stage 2 2% faster than at the tail, 3%
+// faster than first).
 template <class T> constexpr bool kEarlyForce = LinkSlots<T>::n == 9;
 
 // One fused Luescher RK3 stage for one link (reference src/flow/gradient_flow.cpp:15-128):
@@ -922,15 +924,12 @@ __device__ __forceinline__ void flow_tail(const LatDev<T>& L, const M3<T>& W, T*
     if (STAGE == 1) {
         st_a9(zs, L.S, mu, p, i, Z);
         X = a9_scale(T(0.25) * eps, Z);
+    } else if (STAGE == 2) {
+        A9<T> Z1 = a9_axpby(T(8.0 / 9.0), Z, T(-17.0 / 36.0), Z0);
+        st_a9(zs, L.S, mu, p, i, Z1);
+        X = a9_scale(eps, Z1);
     } else {
-        const A9<T> Zp = kEarlyForce<T> ? Z0 : ld_a9((const T*)zs, L.S, mu, p, i);
-        if (STAGE == 2) {
-            A9<T> Z1 = a9_axpby(T(8.0 / 9.0), Z, T(-17.0 / 36.0), Zp);
-            st_a9(zs, L.S, mu, p, i, Z1);
-            X = a9_scale(eps, Z1);
-        } else {
-            X = a9_axpby(T(0.75) * eps, Z, -eps, Zp);
-        }
+        X = a9_axpby(T(0.75) * eps, Z, -eps, Z0);
     }
     M3<T> E = exp_su3_ah<T, ExpTerms<T>::N>(X);
     M3<T> Wn = mul_nn(E, W);
@@ -1156,7 +1155,7 @@ template <class T, int STAGE, bool XL>
 __device__ __forceinline__ void flow_stage_body(const LatDev<T>& L, const T* __restrict__ win, T* __restrict__ wout,
                                                 T* __restrict__ zs, bool live, int mu, Coords c, int p, uint32_t i,
                                                 uint32_t s, FlowLds<T>& sh, T eps) {
-    A9<T> Z0;   // the stored force, loaded here in FP64 (kEarlyForce)
+    A9<T> Z0;   // the stored force (stages 2, 3)
     if (kEarlyForce<T> && STAGE != 1 && live) Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
     M3<T> Sg;
     if (live) {
@@ -1168,6 +1167,7 @@ __device__ __forceinline__ void flow_stage_body(const LatDev<T>& L, const T* __r
         }
     }
     __syncthreads();
+    if (!kEarlyForce<T> && STAGE != 1 && live) Z0 = ld_a9((const T*)zs, L.S, mu, p, i);
     if (live) {
         switch (mu) {
             case 0: flow_phase2<T, 0, STAGE, XL>(L, win, wout, zs, Z0, c, p, i, s, sh, Sg, eps); break;
