@@ -880,7 +880,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         Coords c = tiled_coords(L, q, p);
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
         M3<T> A = staple<T, MU>(L, L.U, c, p, i);
-        M3<T> U = ld_link(L.U, L.S, MU, p, i);
+        M3<T> U = ld_link<T, kLinkOrder, true>(L.U, L.S, MU, p, i);
         const T beta = (T)L.beta;
         const uint32_t sw = sweep + L.sweep_off;
         int ex = 0;

@@ -238,7 +238,7 @@ __device__ __forceinline__ size_t link_base(uint32_t S, int hs, uint32_t i) {
 }
 
 // The stored complex entries of a link (rows 0-2 in FP64, rows 0-1 in FP32).
-template <class T, int LAY>
+template <class T, int LAY, bool NT = false>
 __device__ __forceinline__ M3<T> ld_stored(const T* f, uint32_t S, int mu, int p, uint32_t i) {
     typedef SlotOf<T, LAY> SO;
     typedef typename SO::type V;
@@ -247,16 +247,16 @@ __device__ __forceinline__ M3<T> ld_stored(const T* f, uint32_t S, int mu, int p
     M3<T> m;
 #pragma unroll
     for (int j = 0; j < slots; ++j) {
-        V c = b[64 * j];
+        V c = NT ? __builtin_nontemporal_load(b + 64 * j) : b[64 * j];
 #pragma unroll
         for (int e = 0; e < w; ++e) m.v[w * j + e] = c[e];
     }
     return m;
 }
 
-template <class T, int LAY = kLinkOrder>
+template <class T, int LAY = kLinkOrder, bool NT = false>
 __device__ __forceinline__ M3<T> ld_link(const T* f, uint32_t S, int mu, int p, uint32_t i) {
-    M3<T> m = ld_stored<T, LAY>(f, S, mu, p, i);
+    M3<T> m = ld_stored<T, LAY, NT>(f, S, mu, p, i);
     if (LinkSlots<T>::n == 6) su3_third_row(m);
     return m;
 }
