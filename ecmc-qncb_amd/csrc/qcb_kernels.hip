@@ -880,6 +880,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
         Coords c = tiled_coords(L, q, p);
         const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
         M3<T> A = staple<T, MU>(L, L.U, c, p, i);
+        // no other thread of the half-sweep reads a link of its own class (the
+        // staples of (mu, p) use (mu, 1-p) and the other directions), so the
+        // updated link is loaded non-temporally, like it is stored
         M3<T> U = ld_link<T, kLinkOrder, true>(L.U, L.S, MU, p, i);
         const T beta = (T)L.beta;
         const uint32_t sw = sweep + L.sweep_off;
