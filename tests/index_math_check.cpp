@@ -53,8 +53,14 @@ static Coords plain_tiled(const LatDev<float>& L, uint32_t q, int p) {
     return c;
 }
 
+// slot of a link in the AoSoA-64 storage: class hs of S links, 64-link
+// blocks of n slots, entry-major inside a block
+static size_t plain_slot(uint32_t n, uint32_t S, int hs, uint32_t i) {
+    return ((size_t)hs * (S / 64) + i / 64) * (64 * (size_t)n) + i % 64;
+}
+
 int main() {
-    long bad_div = 0, n_div = 0, bad_nb = 0, n_nb = 0, bad_tc = 0, n_tc = 0;
+    long bad_div = 0, n_div = 0, bad_nb = 0, n_nb = 0, bad_tc = 0, n_tc = 0, bad_sl = 0, n_sl = 0;
     std::mt19937_64 g(1);
     for (uint32_t d = 1; d < 70000; ++d) {
         const FastDiv f = make_fastdiv(d);
@@ -95,7 +101,19 @@ int main() {
                         }
                 }
         }
-    printf("fdiv %ld/%ld wrong, nb_cb %ld/%ld wrong, tiled_coords %ld/%ld wrong\n", bad_div, n_div, bad_nb, n_nb,
-           bad_tc, n_tc);
-    return bad_div || bad_nb || bad_tc;
+    // aosoa_slot up to the largest lattice (4V < 2^31, so S <= 2^28) and the
+    // largest slot count (9), every class of links (8) and of plane fields
+    for (uint32_t n : {2u, 3u, 6u, 9u})
+        for (uint32_t S : {64u, 192u, 1u << 20, (1u << 27) + 64u, 1u << 28})
+            for (int hs = 0; hs < 12; ++hs) {
+                const uint32_t is[6] = {0u, 1u, 63u, 64u, S - 1u, (uint32_t)(g() % S)};
+                for (uint32_t i : is) ++n_sl, bad_sl += aosoa_slot(n, S, hs, i) != plain_slot(n, S, hs, i);
+                for (int k = 0; k < 2000; ++k) {
+                    const uint32_t i = (uint32_t)(g() % S);
+                    ++n_sl, bad_sl += aosoa_slot(n, S, hs, i) != plain_slot(n, S, hs, i);
+                }
+            }
+    printf("fdiv %ld/%ld wrong, nb_cb %ld/%ld wrong, tiled_coords %ld/%ld wrong, aosoa_slot %ld/%ld wrong\n", bad_div,
+           n_div, bad_nb, n_nb, bad_tc, n_tc, bad_sl, n_sl);
+    return bad_div || bad_nb || bad_tc || bad_sl;
 }

@@ -1,7 +1,9 @@
 """The device index arithmetic, built for the host: closed-form periodic
-neighbour steps, multiply-high division by the lattice's invariant divisors
-and the tiled work order, each against plain integer arithmetic over whole
-lattices (extents 1 and 2 included) and against x / d for 2e6 numerators."""
+neighbour steps, multiply-high division by the lattice's invariant divisors,
+the tiled work order and the AoSoA slot of a link (its 32-bit offset inside a
+class up to the largest lattice), each against plain integer arithmetic over
+whole lattices (extents 1 and 2 included) and against x / d for 2e6
+numerators."""
 import os
 import shutil
 import subprocess
