@@ -417,7 +417,7 @@ extern "C" int qcb_translate(qcb_lattice* lat, const int s[4]) {
 // ----------------------------------------------------------------- flow ---
 
 int enqueue_flow_step(qcb_lattice* lat, double eps) {
-    Launch g = lat_launch(lat, kFamLink);
+    Launch g = lat_launch(lat, kFamFlow);
     for (int st = 1; st <= 3; ++st) {
         if (lat->prec == QCB_F64) launch_flow_stage<double>(g, st, lat->gstep, eps);
         else launch_flow_stage<float>(g, st, lat->gstep, eps);

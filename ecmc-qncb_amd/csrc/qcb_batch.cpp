@@ -30,8 +30,8 @@ struct qcb_batch {
     int n = 0;
     std::vector<qcb_lattice*> lats;
     void* d_desc = nullptr;
-    uint32_t* d_blk[3] = {nullptr, nullptr, nullptr};
-    uint32_t nb[3] = {0, 0, 0};
+    uint32_t* d_blk[kNumFam] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t nb[kNumFam] = {0, 0, 0, 0};
     double* d_partial = nullptr;     // 2 * nb[kFamSite]
     double* d_meas = nullptr;        // measurement buffer (grown on demand)
     size_t meas_cap = 0;
@@ -139,7 +139,7 @@ void batch_free(qcb_batch* b) {
     for (qcb_lattice* l : b->lats)
         if (l) qcb_lattice_destroy(l);
     (void)hipFree(b->d_desc);
-    for (int f = 0; f < 3; ++f) (void)hipFree(b->d_blk[f]);
+    for (int f = 0; f < kNumFam; ++f) (void)hipFree(b->d_blk[f]);
     (void)hipFree(b->d_partial);
     (void)hipFree(b->d_meas);
     (void)hipFree(b->d_sweep);
@@ -185,7 +185,7 @@ int set_groups(qcb_batch* b, int n) {
         const auto& idx = b->grp_idx[g];
         QCB_HIP(c, hipMalloc(&b->grp_desc[g], idx.size() * sizeof(LatDev<double>)));
         std::vector<uint32_t> pre(idx.size() + 1, 0);
-        for (size_t k = 0; k < idx.size(); ++k) pre[k + 1] = pre[k] + fam_blocks(b->lats[idx[k]]->Vh, kFamHalf);
+        for (size_t k = 0; k < idx.size(); ++k) pre[k + 1] = pre[k] + fam_blocks(*b->lats[idx[k]], kFamHalf);
         b->grp_nb[g] = pre.back();
         QCB_HIP(c, hipMalloc((void**)&b->grp_blk[g], pre.size() * sizeof(uint32_t)));
         QCB_HIP(c, hipMemcpy(b->grp_blk[g], pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -266,7 +266,8 @@ int zero_counters(qcb_batch* b) {
 
 // Every address a flow launch sequence bakes into its kernel arguments.
 std::vector<const void*> flow_key(const qcb_batch* b) {
-    std::vector<const void*> k = {b->d_meas, b->d_partial, b->d_desc, b->d_blk[0], b->d_blk[1], b->d_blk[2]};
+    std::vector<const void*> k = {b->d_meas, b->d_partial, b->d_desc};
+    for (const uint32_t* p : b->d_blk) k.push_back(p);
     for (const qcb_lattice* l : b->lats) {
         k.push_back(l->U);
         for (void* f : l->F) k.push_back(f);
@@ -337,9 +338,9 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
         }
     }
     QCB_HIP(ctx, hipMalloc(&b->d_desc, (size_t)n * sizeof(LatDev<double>)));
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < kNumFam; ++f) {
         std::vector<uint32_t> pre(n + 1, 0);
-        for (int i = 0; i < n; ++i) pre[i + 1] = pre[i] + fam_blocks(b->lats[i]->Vh, f);
+        for (int i = 0; i < n; ++i) pre[i + 1] = pre[i] + fam_blocks(*b->lats[i], f);
         b->nb[f] = pre[n];
         QCB_HIP(ctx, hipMalloc((void**)&b->d_blk[f], pre.size() * sizeof(uint32_t)));
         QCB_HIP(ctx, hipMemcpy(b->d_blk[f], pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -437,7 +438,7 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
     if ((rc = batch_upload(b))) return rc;
     const bool f64 = b->prec == QCB_F64;
     auto enqueue = [&]() -> int {
-        Launch gl = batch_launch(b, kFamLink), gs = batch_launch(b, kFamSite);
+        Launch gl = batch_launch(b, kFamFlow), gs = batch_launch(b, kFamSite);
         double* pq = b->d_partial;
         double* pe = b->d_partial + b->nb[kFamSite];
         // copy links into the flow fields of every lattice: one launch per lattice
@@ -504,7 +505,7 @@ extern "C" int qcb_batch_flow_steps(qcb_batch* b, double eps, int n_steps) {
             if (f64) launch_copy_field<double>(one, (const double*)l->U, (double*)l->F[0], kLinkOrder, kFlowOrder);
             else launch_copy_field<float>(one, (const float*)l->U, (float*)l->F[0], kLinkOrder, kFlowOrder);
         }
-        Launch gl = batch_launch(b, kFamLink);
+        Launch gl = batch_launch(b, kFamFlow);
         for (int s = 0; s < n_steps; ++s)
             for (int st = 1; st <= 3; ++st) {
                 if (f64) launch_flow_stage<double>(gl, st, (uint32_t)s, eps);

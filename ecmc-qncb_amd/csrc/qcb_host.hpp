@@ -19,7 +19,8 @@ struct qcb_ctx {
 };
 
 namespace qcb {
-enum Family { kFamHalf = 0, kFamSite = 1, kFamLink = 2 };
+enum Family { kFamHalf = 0, kFamSite = 1, kFamLink = 2, kFamFlow = 3 };
+constexpr int kNumFam = 4;
 }
 
 struct qcb_lattice {
@@ -31,8 +32,8 @@ struct qcb_lattice {
     void* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
     void* d_desc = nullptr;
-    uint32_t* d_blk[3] = {nullptr, nullptr, nullptr};
-    uint32_t nb[3] = {0, 0, 0};
+    uint32_t* d_blk[qcb::kNumFam] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t nb[qcb::kNumFam] = {0, 0, 0, 0};
     double* d_partial = nullptr;   // 2 * nb[kFamSite] doubles
     double* d_out = nullptr;       // 8 doubles of results
     unsigned long long* d_counter = nullptr;
@@ -101,9 +102,12 @@ inline void default_tiling(const int L[4], int tile[4], int* morton) {
 }
 
 // Blocks one lattice contributes to a launch of the given kernel family.
-inline uint32_t fam_blocks(uint32_t Vh, int fam) {
-    const uint32_t per = fam == kFamHalf ? kHalfSites : fam == kFamSite ? kSiteSites : kLinkSites;
-    return (Vh + per - 1) / per;
+inline uint32_t fam_blocks(const qcb_lattice& l, int fam) {
+    const uint32_t per = fam == kFamHalf   ? kHalfSites
+                         : fam == kFamSite ? kSiteSites
+                         : fam == kFamFlow ? flow_block_sites((uint32_t)l.L[0] / 2u)
+                                           : kLinkSites;
+    return (l.Vh + per - 1) / per;
 }
 
 // The multiplier of FastDiv for divisor d >= 1.
@@ -142,6 +146,7 @@ inline LatDev<T> make_desc(const qcb_lattice* l) {
     d.kp_cap = l->kp_cap;
     d.kp_exh = l->kp_exh;
     d.flow_limit = l->flow_limit;
+    d.fsb = flow_block_sites((uint32_t)l->L[0] / 2u);
     d.qe_out = l->qe_out;
     d.tbits[0] = 0;
     for (int k = 1; k < 4; ++k) {
@@ -186,8 +191,8 @@ inline int lattice_alloc(qcb_lattice* l) {
     if (hipMalloc(&l->U, field_bytes(l)) != hipSuccess)
         return qcb_set_error(c, QCB_ENOMEM, "cannot allocate %llu bytes of links", (unsigned long long)field_bytes(l));
     QCB_HIP(c, hipMalloc(&l->d_desc, sizeof(LatDev<double>)));
-    for (int f = 0; f < 3; ++f) {
-        l->nb[f] = fam_blocks(l->Vh, f);
+    for (int f = 0; f < kNumFam; ++f) {
+        l->nb[f] = fam_blocks(*l, f);
         uint32_t h[2] = {0, l->nb[f]};
         QCB_HIP(c, hipMalloc((void**)&l->d_blk[f], sizeof h));
         QCB_HIP(c, hipMemcpy(l->d_blk[f], h, sizeof h, hipMemcpyHostToDevice));
@@ -236,7 +241,7 @@ inline void lattice_free(qcb_lattice* l) {
     (void)hipFree(l->U);
     for (int k = 0; k < 3; ++k) (void)hipFree(l->F[k]);
     (void)hipFree(l->d_desc);
-    for (int f = 0; f < 3; ++f) (void)hipFree(l->d_blk[f]);
+    for (int f = 0; f < kNumFam; ++f) (void)hipFree(l->d_blk[f]);
     (void)hipFree(l->d_partial);
     (void)hipFree(l->d_out);
     (void)hipFree(l->d_counter);

@@ -1188,14 +1188,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
     if (gstep - L.flow_start >= L.flow_limit) return;   // the whole block: its lattice has finished flowing
+    // a block holds L.fsb sites (whole x rows, or 32 sites of partial rows);
+    // site lanes from L.fsb on are idle
     const uint32_t s = threadIdx.x & (kLinkSites - 1);
-    const uint32_t q = bc.local * kLinkSites + s;
+    const uint32_t q = bc.local * L.fsb + s;
     // wavefront w of the block takes direction (w + block) mod 4, so that
     // each SIMD runs every direction's share of the work
     const uint32_t hs = (((threadIdx.x >> 6) + blockIdx.x) & 3u) * 2u + ((threadIdx.x >> 5) & 1u);
     // every thread reaches the barrier; a site past the end has all four of
     // its links past the end, so no live thread reads what it did not write
-    const bool live = q < L.Vh;
+    const bool live = s < L.fsb && q < L.Vh;
     const int mu = (int)(hs >> 1), p = (int)(hs & 1);
     const int cur = (int)((gstep - L.flow_start) & 1u);
     const T* win = L.F[(STAGE == 2) ? 1 - cur : cur];
@@ -1203,9 +1205,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     T* zs = L.F[2];
     Coords c = tiled_coords(L, live ? q : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
-    // whole x rows per block (Lx/2 divides the block's 32 sites): the
-    // (mu, 0) lower staples come from LDS; the branch is uniform per block
-    if ((kLinkSites % (uint32_t)L.Lxh) == 0u)
+    // whole x rows per block (Lx/2 divides its sites; the rows start at q =
+    // 0 and every work-order tile is whole rows): the (mu, 0) lower staples
+    // come from LDS; the branch is uniform per block
+    if ((L.fsb % (uint32_t)L.Lxh) == 0u)
         flow_stage_body<T, STAGE, true>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);
     else
         flow_stage_body<T, STAGE, false>(L, win, wout, zs, live, mu, c, p, i, s, sh, eps);

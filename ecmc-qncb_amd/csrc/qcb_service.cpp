@@ -124,8 +124,8 @@ struct Arena {
 // Descriptors and block prefix sums of one stream's lattices, on the device.
 struct DevSet {
     void* d_desc = nullptr;
-    uint32_t* d_blk[3] = {nullptr, nullptr, nullptr};
-    uint32_t nb[3] = {0, 0, 0};
+    uint32_t* d_blk[kNumFam] = {nullptr, nullptr, nullptr, nullptr};
+    uint32_t nb[kNumFam] = {0, 0, 0, 0};
     int n = 0, cap = 0;
     char* stage[2] = {nullptr, nullptr};     // pinned staging, used alternately
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -161,11 +161,11 @@ double now_s(const qcb_service* s) {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - s->t0).count();
 }
 
-size_t set_stage_bytes(int cap) { return (size_t)cap * sizeof(LatDev<double>) + 3 * (size_t)(cap + 1) * 4; }
+size_t set_stage_bytes(int cap) { return (size_t)cap * sizeof(LatDev<double>) + kNumFam * (size_t)(cap + 1) * 4; }
 
 void devset_free(DevSet& d) {
     (void)hipFree(d.d_desc);
-    for (int f = 0; f < 3; ++f) (void)hipFree(d.d_blk[f]);
+    for (int f = 0; f < kNumFam; ++f) (void)hipFree(d.d_blk[f]);
     for (int k = 0; k < 2; ++k) {
         if (d.stage[k]) (void)hipHostFree(d.stage[k]);
         if (d.stage_ev[k]) (void)hipEventDestroy(d.stage_ev[k]);
@@ -173,7 +173,7 @@ void devset_free(DevSet& d) {
         d.stage_ev[k] = nullptr;
     }
     d.d_desc = nullptr;
-    for (int f = 0; f < 3; ++f) d.d_blk[f] = nullptr;
+    for (int f = 0; f < kNumFam; ++f) d.d_blk[f] = nullptr;
     d.cap = 0;
 }
 
@@ -197,7 +197,7 @@ int devset_upload(qcb_service* s, DevSet& d, const std::vector<Req*>& reqs, hipS
     d.dirty = false;
     d.n = (int)reqs.size();
     if (d.n == 0) {
-        d.nb[0] = d.nb[1] = d.nb[2] = 0;
+        for (int f = 0; f < kNumFam; ++f) d.nb[f] = 0;
         return QCB_OK;
     }
     if (d.n > d.cap) {   // rare: replace the arrays once nothing reads them
@@ -206,7 +206,7 @@ int devset_upload(qcb_service* s, DevSet& d, const std::vector<Req*>& reqs, hipS
         devset_free(d);
         d.cap = std::max({d.n, 2 * d.cap, 64});
         QCB_HIP(c, hipMalloc(&d.d_desc, (size_t)d.cap * sizeof(LatDev<double>)));
-        for (int f = 0; f < 3; ++f) QCB_HIP(c, hipMalloc((void**)&d.d_blk[f], (size_t)(d.cap + 1) * 4));
+        for (int f = 0; f < kNumFam; ++f) QCB_HIP(c, hipMalloc((void**)&d.d_blk[f], (size_t)(d.cap + 1) * 4));
         for (int k = 0; k < 2; ++k) {
             QCB_HIP(c, hipHostMalloc((void**)&d.stage[k], set_stage_bytes(d.cap), hipHostMallocDefault));
             QCB_HIP(c, hipEventCreateWithFlags(&d.stage_ev[k], hipEventDisableTiming));
@@ -219,10 +219,10 @@ int devset_upload(qcb_service* s, DevSet& d, const std::vector<Req*>& reqs, hipS
     if (s->prec == QCB_F64) stage_descs<double>(h, reqs);
     else stage_descs<float>(h, reqs);
     uint32_t* pre = reinterpret_cast<uint32_t*>(h + (size_t)d.cap * sizeof(LatDev<double>));
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < kNumFam; ++f) {
         uint32_t* p = pre + (size_t)f * (d.cap + 1);
         p[0] = 0;
-        for (int i = 0; i < d.n; ++i) p[i + 1] = p[i] + fam_blocks(reqs[i]->lat.Vh, f);
+        for (int i = 0; i < d.n; ++i) p[i + 1] = p[i] + fam_blocks(reqs[i]->lat, f);
         d.nb[f] = p[d.n];
         QCB_HIP(c, hipMemcpyAsync(d.d_blk[f], p, (size_t)(d.n + 1) * 4, hipMemcpyHostToDevice, st));
     }
@@ -520,7 +520,7 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
             launch_clover<float>(gs, 1, it, pq, pe);
             launch_finalize_qe<float>(gs, pq, pe, nullptr, 0, it);
         }
-        const Launch gl = set_launch(s->set_flow, kFamLink, s->s_fl);
+        const Launch gl = set_launch(s->set_flow, kFamFlow, s->s_fl);
         for (int st = 1; st <= 3; ++st) {
             if (f64) launch_flow_stage<double>(gl, st, it, s->eps);
             else launch_flow_stage<float>(gl, st, it, s->eps);

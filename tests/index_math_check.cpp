@@ -113,7 +113,15 @@ int main() {
                     ++n_sl, bad_sl += aosoa_slot(n, S, hs, i) != plain_slot(n, S, hs, i);
                 }
             }
-    printf("fdiv %ld/%ld wrong, nb_cb %ld/%ld wrong, tiled_coords %ld/%ld wrong, aosoa_slot %ld/%ld wrong\n", bad_div,
-           n_div, bad_nb, n_nb, bad_tc, n_tc, bad_sl, n_sl);
-    return bad_div || bad_nb || bad_tc || bad_sl;
+    // flow-stage block sites: at most 32; whole rows exactly when 30 or more
+    // of the 32 site lanes can hold whole rows
+    long bad_fb = 0;
+    for (uint32_t lxh = 2; lxh <= 80; ++lxh) {
+        const uint32_t fb = flow_block_sites(lxh), best = lxh <= 32 ? lxh * (32 / lxh) : 0;
+        bad_fb += fb > 32 || (best >= 30 ? fb != best : fb != 32);
+    }
+    printf("fdiv %ld/%ld wrong, nb_cb %ld/%ld wrong, tiled_coords %ld/%ld wrong, aosoa_slot %ld/%ld wrong, "
+           "flow_block_sites %ld/79 wrong\n",
+           bad_div, n_div, bad_nb, n_nb, bad_tc, n_tc, bad_sl, n_sl, bad_fb);
+    return bad_div || bad_nb || bad_tc || bad_sl || bad_fb;
 }

@@ -383,9 +383,12 @@ def test_reunitarize_and_translate(ctx):
     assert np.array_equal(g.get_links(), o.links)
 
 
-# (32, 8, 4, 4): whole x rows per block, so the flow takes its (mu, 0) lower
-# staples and the clover its x- and y-planes' shared leaves from LDS
-@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (10, 6, 4, 8), (32, 8, 4, 4)])
+# The flow stage's blocks hold whole x rows for Lx/2 = 3 and 5 (30 of the 32
+# site lanes) and 16 (all 32): those lattices take the (mu, 0) lower staples
+# from LDS; Lx/2 = 7 leaves blocks of 32 sites in partial rows, which form
+# every lower staple from the field.  (32, 8, 4, 4) also gives the clover
+# its x- and y-planes' shared leaves.
+@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (10, 6, 4, 8), (14, 4, 4, 6), (32, 8, 4, 4)])
 def test_flow_matches_oracle(ctx, dims):
     g, o = hot_pair(ctx, dims, 17)
     for s in range(2):

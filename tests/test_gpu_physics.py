@@ -153,14 +153,17 @@ def test_fp32_against_fp64_same_inputs(ctx):
     assert (np.abs(a[:, 2] - b[:, 2]) / a[:, 2]).max() < 1e-5
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (64, 4, 4, 4), (12, 6, 6, 8), (24, 4, 4, 6), (6, 4, 4, 4)])
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (64, 4, 4, 4), (12, 6, 6, 8), (24, 4, 4, 6), (6, 4, 4, 4),
+                                  (14, 4, 4, 6)])
 def test_fp32_flow_links_against_fp64(ctx, dims):
     """The FP32 flow stage takes the lower staples of the planes (mu, 0),
-    mu != 0, from the block's shared memory when a block holds whole x rows
-    (Lx/2 divides 32: 8^4 and 64x4^3) and from the field otherwise (Lx = 12,
-    24, 6).  Both paths against the FP64 flow of the same links: every link
-    after 10 RK3 steps at eps = 0.02 within 1e-5 (float rounding; a wrong
-    staple would be off by O(eps) ~ 1e-2)."""
+    mu != 0, from the block's shared memory when a block holds whole x rows:
+    32 sites when Lx/2 divides 32 (8^4 and 64x4^3), 30 sites with two idle
+    lanes for Lx/2 = 6 and 3 (Lx = 12, 6); with Lx/2 = 12 or 7 (Lx = 24, 14)
+    its 32 sites are partial rows and every lower staple comes from the field.
+    All paths against the FP64 flow of the same links: every link after 10
+    RK3 steps at eps = 0.02 within 1e-5 (float rounding; a wrong staple would
+    be off by O(eps) ~ 1e-2)."""
     d = thermal_links(ctx, dims, sweeps=3, seed=5)
     s = qcb.Lattice(ctx, dims, qcb.F32)
     s.set_links(d.get_links())
