@@ -9,9 +9,12 @@ import qcb
 
 pytestmark = pytest.mark.gpu
 
-DIMS = [(4, 4, 4, 4), (6, 6, 6, 6), (8, 4, 6, 4), (4, 4, 4, 8), (10, 10, 10, 10)]
-BETAS = [5.8, 6.0, 6.2, 5.9, 6.1]
-SEEDS = [3, 5, 7, 11, 13]
+# the flow stage's blocks: 32 sites in whole rows (Lx = 4, 8), 30 sites in
+# whole rows (Lx = 6, 10, 20) and 32 sites in partial rows (Lx = 14), all in
+# one launch
+DIMS = [(4, 4, 4, 4), (6, 6, 6, 6), (8, 4, 6, 4), (4, 4, 4, 8), (10, 10, 10, 10), (14, 4, 4, 6), (20, 4, 4, 4)]
+BETAS = [5.8, 6.0, 6.2, 5.9, 6.1, 6.0, 5.95]
+SEEDS = [3, 5, 7, 11, 13, 17, 19]
 
 
 @pytest.fixture(scope="module")
