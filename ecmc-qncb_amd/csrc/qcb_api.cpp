@@ -276,9 +276,9 @@ int enqueue_plaquette(qcb_lattice* lat, int which, double* d_out) {
 }
 
 int enqueue_clover(qcb_lattice* lat, int which, double* d_out) {
-    Launch g = lat_launch(lat, kFamSite);
+    Launch g = lat_launch(lat, kFamClover);
     double* pq = lat->d_partial;
-    double* pe = lat->d_partial + lat->nb[kFamSite];
+    double* pe = lat->d_partial + lat->nb[kFamClover];
     if (lat->prec == QCB_F64) {
         launch_clover<double>(g, which, lat->gstep, pq, pe);
         launch_finalize_qe<double>(g, pq, pe, d_out, 2);

@@ -30,9 +30,9 @@ struct qcb_batch {
     int n = 0;
     std::vector<qcb_lattice*> lats;
     void* d_desc = nullptr;
-    uint32_t* d_blk[kNumFam] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t nb[kNumFam] = {0, 0, 0, 0};
-    double* d_partial = nullptr;     // 2 * nb[kFamSite]
+    uint32_t* d_blk[kNumFam] = {};
+    uint32_t nb[kNumFam] = {};
+    double* d_partial = nullptr;     // 2 * nb[kFamClover] (>= 2 * nb[kFamSite])
     double* d_meas = nullptr;        // measurement buffer (grown on demand)
     size_t meas_cap = 0;
     uint32_t* d_sweep = nullptr;     // device sweep counter for graph replay
@@ -345,7 +345,7 @@ extern "C" int qcb_batch_create(qcb_ctx* ctx, int n, const int* dims, int prec, 
         QCB_HIP(ctx, hipMalloc((void**)&b->d_blk[f], pre.size() * sizeof(uint32_t)));
         QCB_HIP(ctx, hipMemcpy(b->d_blk[f], pre.data(), pre.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     }
-    QCB_HIP(ctx, hipMalloc((void**)&b->d_partial, 2 * sizeof(double) * b->nb[kFamSite]));
+    QCB_HIP(ctx, hipMalloc((void**)&b->d_partial, 2 * sizeof(double) * b->nb[kFamClover]));
     QCB_HIP(ctx, hipMalloc((void**)&b->d_sweep, sizeof(uint32_t)));
     QCB_HIP(ctx, hipMemset(b->d_sweep, 0, sizeof(uint32_t)));
     rc = batch_upload(b);
@@ -438,9 +438,9 @@ extern "C" int qcb_batch_flow(qcb_batch* b, double eps, int n_steps, double* tQE
     if ((rc = batch_upload(b))) return rc;
     const bool f64 = b->prec == QCB_F64;
     auto enqueue = [&]() -> int {
-        Launch gl = batch_launch(b, kFamFlow), gs = batch_launch(b, kFamSite);
+        Launch gl = batch_launch(b, kFamFlow), gs = batch_launch(b, kFamClover);
         double* pq = b->d_partial;
-        double* pe = b->d_partial + b->nb[kFamSite];
+        double* pe = b->d_partial + b->nb[kFamClover];
         // copy links into the flow fields of every lattice: one launch per lattice
         for (qcb_lattice* l : b->lats) {
             Launch one = lat_launch(l, kFamLink);

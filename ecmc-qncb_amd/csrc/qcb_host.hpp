@@ -19,8 +19,8 @@ struct qcb_ctx {
 };
 
 namespace qcb {
-enum Family { kFamHalf = 0, kFamSite = 1, kFamLink = 2, kFamFlow = 3 };
-constexpr int kNumFam = 4;
+enum Family { kFamHalf = 0, kFamSite = 1, kFamLink = 2, kFamFlow = 3, kFamClover = 4 };
+constexpr int kNumFam = 5;
 }
 
 struct qcb_lattice {
@@ -32,9 +32,9 @@ struct qcb_lattice {
     void* U = nullptr;
     void* F[3] = {nullptr, nullptr, nullptr};
     void* d_desc = nullptr;
-    uint32_t* d_blk[qcb::kNumFam] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t nb[qcb::kNumFam] = {0, 0, 0, 0};
-    double* d_partial = nullptr;   // 2 * nb[kFamSite] doubles
+    uint32_t* d_blk[qcb::kNumFam] = {};
+    uint32_t nb[qcb::kNumFam] = {};
+    double* d_partial = nullptr;   // 2 * nb[kFamClover] doubles (>= 2 * nb[kFamSite])
     double* d_out = nullptr;       // 8 doubles of results
     unsigned long long* d_counter = nullptr;
     double beta = 0.0;
@@ -106,7 +106,8 @@ inline uint32_t fam_blocks(const qcb_lattice& l, int fam) {
     const uint32_t per = fam == kFamHalf   ? kHalfSites
                          : fam == kFamSite ? kSiteSites
                          : fam == kFamFlow ? flow_block_sites((uint32_t)l.L[0] / 2u)
-                                           : kLinkSites;
+                         : fam == kFamClover ? 4u * flow_block_sites((uint32_t)l.L[0] / 2u)
+                                             : kLinkSites;
     return (l.Vh + per - 1) / per;
 }
 
@@ -197,7 +198,7 @@ inline int lattice_alloc(qcb_lattice* l) {
         QCB_HIP(c, hipMalloc((void**)&l->d_blk[f], sizeof h));
         QCB_HIP(c, hipMemcpy(l->d_blk[f], h, sizeof h, hipMemcpyHostToDevice));
     }
-    QCB_HIP(c, hipMalloc((void**)&l->d_partial, 2 * sizeof(double) * l->nb[kFamSite]));
+    QCB_HIP(c, hipMalloc((void**)&l->d_partial, 2 * sizeof(double) * l->nb[kFamClover]));
     QCB_HIP(c, hipMalloc((void**)&l->d_out, 8 * sizeof(double)));
     QCB_HIP(c, hipMalloc((void**)&l->d_counter, sizeof(unsigned long long)));
     QCB_HIP(c, hipMemset(l->d_counter, 0, sizeof(unsigned long long)));

@@ -491,10 +491,12 @@ __device__ __forceinline__ double sqnorm(const M3<T>& a) {
 // Q3(x) = V^dag Q4(y) V, so with H = (Q1 + Q4) - (Q1 + Q4)^dag, the
 // anti-Hermitian part that F needs,
 //   8 F_mu,nu(x) = H(x) + V^dag H(x - mu) V.
-// A wavefront holds 32 cb sites of both parities (lanes 0-31 parity 0,
-// 32-63 parity 1, the same 32 cb sites), and a block 128 cb sites.  They
-// are whole x rows whenever Lx/2 divides 32 (the work order runs x fastest
-// over whole rows), and consecutive y rows of a tile follow each other.
+// A wavefront holds up to 32 cb sites of both parities (lanes 0-31 parity
+// 0, 32-63 parity 1, the same cb sites), and a block four wavefronts' sites.
+// They are whole x rows when Lx/2 divides 32, and also when Lx/2 = 3, 5, 6,
+// 10, 15, whose wavefronts hold 30 sites (flow_block_sites; the work order
+// runs x fastest over whole rows), and consecutive y rows of a tile follow
+// each other.
 // Every site then forms Q1 + Q4 of the planes (0, nu) and (1, 2), (1, 3) and
 // publishes their H in LDS; after a barrier a site takes F of the planes
 // (0, nu) from its x-neighbour's H (always in the block), and F of (1, 2),
@@ -513,12 +515,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     __shared__ uint32_t cbs[kSiteSites];             // cb index of each slot (both parities share it)
     BlockCtx<T> bc = block_ctx(lats, blk, n_lat, nblocks);
     const LatDev<T>& L = *bc.L;
+    // each wavefront holds L.fsb cb sites (of both parities): whole x rows
+    // when Lx/2 divides 32 or when whole rows fill 30 of its 32 site lanes
+    // (the other lanes idle), else 32 sites of partial rows; LDS slot s
+    // keeps 32 slots per wavefront
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t s = (threadIdx.x >> 6) * 32u + (lane & 31u);
-    const uint32_t qs = bc.local * kSiteSites + s;
+    const uint32_t qs = bc.local * 4u * L.fsb + (threadIdx.x >> 6) * L.fsb + (lane & 31u);
     const int p = (int)(lane >> 5);
-    const bool live = qs < L.Vh;   // rows are whole, so a live site's x-neighbour is live
-    const bool xrows = (32u % (uint32_t)L.Lxh) == 0u;   // uniform per block
+    // rows are whole, so a live site's x-neighbour is live
+    const bool live = (lane & 31u) < L.fsb && qs < L.Vh;
+    const bool xrows = (L.fsb % (uint32_t)L.Lxh) == 0u;   // uniform per block
     const Coords c = tiled_coords(L, live ? qs : 0u, p);
     const uint32_t i = cb_index(L, c.x, c.y, c.z, c.t);
     const T* f = LAY == kLinkOrder ? L.U : L.F[(gstep - L.flow_start) & 1u];
@@ -559,9 +566,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(MinWaves
     if (live) {
         const uint32_t step0 = cb_step(L, c, 0, -1);   // cb index and block slot of x - e0
         const uint32_t step1 = cb_step(L, c, 1, -1);   // cb index step to x - e1
-        // x - e1 in the block for every site of the wavefront (its slot is s - Lx/2)
-        const uint32_t sy = s - (uint32_t)L.Lxh;
-        const bool yok = xrows && s >= (uint32_t)L.Lxh && cbs[s >= (uint32_t)L.Lxh ? sy : s] == i + step1;
+        // x - e1 in the block for every site of the wavefront: the previous
+        // row, Lx/2 slots back, or for a wavefront's first row the last row
+        // of the wavefront before, past its 32 - L.fsb idle slots
+        const uint32_t yc = (lane & 31u) < (uint32_t)L.Lxh ? 32u - L.fsb : 0u;
+        const uint32_t sy = s - (uint32_t)L.Lxh - yc;
+        const bool yin = s >= (uint32_t)L.Lxh + yc;
+        const bool yok = xrows && yin && cbs[yin ? sy : s] == i + step1;
         const bool yshare = __all(yok);
         double tq = 0.0;
         // the three plane pairs (01,23), (02,13), (03,12) one at a time: a

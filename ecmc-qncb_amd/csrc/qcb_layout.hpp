@@ -66,7 +66,7 @@ struct LatDev {
     uint32_t kp_cap;      // Kennedy-Pendleton attempts per subgroup draw in Philox mode
     unsigned long long* kp_exh;  // device count of draws that ran out of attempts (skipped updates)
     uint32_t flow_limit;  // RK3 steps this lattice takes after flow_start (later stage launches skip it)
-    uint32_t fsb;         // cb sites per block of the flow stage (flow_block_sites)
+    uint32_t fsb;         // cb sites per 32 site lanes of the flow stage and the clover (flow_block_sites)
     double* qe_out;       // if set, a measurement at flow step s writes (Q, E/V) to qe_out[2s], qe_out[2s+1]
     FastDiv dv_tvol, dv_lxh, dv_t1, dv_t2, dv_nt1, dv_nt2;   // by tvol, Lxh, TL[1], TL[2], NT[1], NT[2]
 };
@@ -84,11 +84,12 @@ struct Gate {
 //   link kernels: 32 cb sites x 8 (mu, parity) classes
 constexpr uint32_t kHalfSites = 256, kSiteSites = 128, kLinkSites = 32;
 
-// cb sites per block of the flow stage (at most kLinkSites): whole x rows of
-// Lx/2 sites when they fill 30 or more of the 32 site lanes, which lets the
-// block take its (mu, 0) lower staples from LDS -- 32 when Lx/2 divides 32,
-// 30 for Lx/2 = 3, 5, 6, 10, 15 (the other lanes idle) -- else 32 sites that
-// need not be whole rows.
+// cb sites per 32 site lanes (a flow-stage block, a clover wavefront), at
+// most kLinkSites: whole x rows of Lx/2 sites when they fill 30 or more of
+// the 32 lanes, which lets the flow stage take its (mu, 0) lower staples and
+// the clover its shared leaves from LDS -- 32 when Lx/2 divides 32, 30 for
+// Lx/2 = 3, 5, 6, 10, 15 (the other lanes idle) -- else 32 sites that need
+// not be whole rows.
 __host__ __device__ inline uint32_t flow_block_sites(uint32_t lxh) {
     const uint32_t rows = lxh <= kLinkSites ? lxh * (kLinkSites / lxh) : 0u;
     return rows + 2u >= kLinkSites ? rows : kLinkSites;

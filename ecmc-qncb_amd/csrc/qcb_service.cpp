@@ -124,8 +124,8 @@ struct Arena {
 // Descriptors and block prefix sums of one stream's lattices, on the device.
 struct DevSet {
     void* d_desc = nullptr;
-    uint32_t* d_blk[kNumFam] = {nullptr, nullptr, nullptr, nullptr};
-    uint32_t nb[kNumFam] = {0, 0, 0, 0};
+    uint32_t* d_blk[kNumFam] = {};
+    uint32_t nb[kNumFam] = {};
     int n = 0, cap = 0;
     char* stage[2] = {nullptr, nullptr};     // pinned staging, used alternately
     hipEvent_t stage_ev[2] = {nullptr, nullptr};
@@ -504,15 +504,15 @@ extern "C" int qcb_service_step(qcb_service* s, int* pending) {
     //    iteration ago
     if (!s->flow.empty()) {
         if (s->n_iter > 0) QCB_HIP(c, hipStreamWaitEvent(s->s_fl, s->ev_hb[(s->n_iter - 1) % kDepth], 0));
-        const Launch gs = set_launch(s->set_flow, kFamSite, s->s_fl);
-        if (2 * (size_t)s->set_flow.nb[kFamSite] > s->partial_cap) {   // rare: more small requests than assumed
+        const Launch gs = set_launch(s->set_flow, kFamClover, s->s_fl);
+        if (2 * (size_t)s->set_flow.nb[kFamClover] > s->partial_cap) {   // rare: more small requests than assumed
             if ((rc = drain_device(s))) return rc;
             (void)hipFree(s->d_partial);
-            s->partial_cap = 4 * (size_t)s->set_flow.nb[kFamSite];
+            s->partial_cap = 4 * (size_t)s->set_flow.nb[kFamClover];
             QCB_HIP(c, hipMalloc((void**)&s->d_partial, s->partial_cap * sizeof(double)));
         }
         double* pq = s->d_partial;
-        double* pe = s->d_partial + s->set_flow.nb[kFamSite];
+        double* pe = s->d_partial + s->set_flow.nb[kFamClover];
         if (f64) {
             launch_clover<double>(gs, 1, it, pq, pe);
             launch_finalize_qe<double>(gs, pq, pe, nullptr, 0, it);

@@ -85,11 +85,13 @@ def test_plaquette_clover_staples(ctx, dims):
     f.close()
 
 
-# Lx/2 in {2, 4, 8, 16, 32} (whole x rows per wavefront) with y tiles of
-# 4, 6 and 8 rows: wavefronts with and without all their y-neighbours in the
-# block, and lattices where no wavefront has them (Lx = 12: no sharing)
+# Lx/2 in {2, 4, 8, 16, 32} (whole x rows in 32-site wavefronts) and
+# {3, 5, 6} (whole rows in 30-site wavefronts, whose first rows find their
+# y-neighbours past the previous wavefront's idle lanes) with y tiles of 4, 6
+# and 8 rows: wavefronts with and without all their y-neighbours in the
+# block; Lx = 14 has no whole rows per wavefront and shares nothing
 @pytest.mark.parametrize("dims", [(4, 8, 4, 6), (8, 4, 8, 4), (16, 6, 4, 4), (32, 4, 6, 4), (64, 4, 4, 4),
-                                  (12, 8, 4, 4)])
+                                  (12, 8, 4, 4), (10, 8, 4, 4), (6, 6, 4, 4), (14, 4, 4, 4)])
 def test_clover_shared_leaves_match_oracle(ctx, dims):
     """The clover's shared leaves (x-planes from the x-neighbour, y-planes
     from the y-neighbour) against the oracle's four-leaf clover, FP64 to
